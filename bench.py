@@ -1,0 +1,162 @@
+"""Benchmark of the fermion hot path on one MI355X.
+
+Workload (one "step"): one M^dagger M solve of the domain-wall operator of
+rhmc/cheisdwf.f at L = Lt = 24, Ls = 32, beta = 0.475, M5 = 1, imass = 1,
+antiperiodic in t, to FP64 true relative residual <= 1e-10, with phi_a ~
+N(0, 1/beta) and a unit-variance complex Gaussian source drawn by the
+reference's own generator (seeded), inputs resident in HBM.
+
+The timed form is the fastest one the library has (mixed precision with FP64
+reliable updates); the plain FP64 GPU solve of the same system is timed
+beside it, and the dslash kernel's HBM rate is measured with HIP events over
+back-to-back launches that cycle through 4 buffer pairs (> the 256 MB
+Infinity Cache).
+
+Prints one JSON line (rank 0, N = 1).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "M†M solve time to 1e-10 true resid, FP64 vs mixed+learned; dslash HBM GB/s"
+HBM_PEAK = 8.0e12
+
+
+def make_inputs(lat, beta, seed=693764523745.0):
+    from chiralheisenberg_amd.rhmc import RandomStream, heatbath_source
+    rng = RandomStream(seed)
+    ps1, ps2 = rng.gaussp(lat.kvol), rng.gaussp(lat.kvol)
+    chi = np.stack([ps1[0], ps1[1], ps2[0]]) / np.sqrt(beta)
+    b = heatbath_source(lat, rng)
+    return chi, b
+
+
+def cpu_baseline(L, Ls, beta, fp64_iters, budget_s=20.0):
+    """Bounded sample of the CPU oracle (C/OpenMP, FP64, all host cores):
+    a few CGNR iterations of the same system, scaled by the FP64 GPU solve's
+    iteration count to the metric's unit (ms per solve)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = min(threads, 16)
+    orc.set_threads(threads)
+    olat = orc.Lattice(L, Ls)
+    rs = np.random.RandomState(3)
+    chi = rs.randn(3, olat.kvol) / np.sqrt(beta)
+    b = (rs.randn(*olat.shape) + 1j * rs.randn(*olat.shape)) / np.sqrt(2)
+    t0 = time.perf_counter()
+    orc.cg_iterations(olat, b, chi, 1)
+    t1 = time.perf_counter() - t0
+    n = max(1, min(20, int(budget_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    orc.cg_iterations(olat, b, chi, n)
+    per_it = (time.perf_counter() - t0) / n
+    return {"value": per_it * fp64_iters * 1e3, "unit": "ms per solve (extrapolated)", "cores": threads,
+            "kind": "port",
+            "sample": "%d CGNR iterations of the FP64 oracle at L=%d Ls=%d (%.1f ms/iteration) x %d iterations of "
+                      "the FP64 GPU solve" % (n, L, Ls, per_it * 1e3, fp64_iters)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--L", type=int, default=24)
+    ap.add_argument("--Ls", type=int, default=32)
+    ap.add_argument("--beta", type=float, default=0.475)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--form", default="fp32", choices=["fp64", "fp32", "bf16"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box of this deployment has one MI355X, and the path "
+              "shards by independent lattice/chain, so --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from chiralheisenberg_amd import fermion as fm
+    lat = fm.Lattice(args.L, args.Ls)
+    chi, b = make_inputs(lat, args.beta)
+    chi_d = fm.DeviceArray.from_numpy(lat, chi)
+    b_d = fm.spinor_dev(lat, b)
+    x_d = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+    zero = np.zeros(lat.shape, np.complex128)
+
+    def solve(form):
+        x_d.upload(zero)
+        lat.synchronize()
+        return fm.solve_mdagm(lat, b_d, x_d, chi_d, args.tol, form)
+
+    print("bench: L=%d Ls=%d form=%s warmup=%d steps=%d" % (args.L, args.Ls, args.form, args.warmup, args.steps),
+          file=sys.stderr, flush=True)
+    # plain FP64 GPU form, once
+    st64 = solve("fp64")
+    print("bench: fp64 solve %.1f ms, %d its, resid %.2e" % (st64["seconds"] * 1e3, st64["iterations"],
+                                                              st64["true_resid"]), file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        solve(args.form)
+    lat.synchronize()
+    t0 = time.perf_counter()
+    stats = [solve(args.form) for _ in range(args.steps)]
+    lat.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
+    st = stats[-1]
+    print("bench: %s solve %.1f ms (wall %.1f ms), %d its, %d reliable updates, resid %.2e" % (
+        args.form, st["seconds"] * 1e3, wall * 1e3, st["iterations"], st["reliable_updates"], st["true_resid"]),
+        file=sys.stderr, flush=True)
+
+    # dslash roofline: FP64 dslash, algorithmic bytes 2V + C
+    kvol = lat.kvol
+    V = lat.kferm * 16
+    C = 3 * kvol * 8
+    ms = fm.time_dslash(lat, "fp64", False, nbuf=4, iters=200)
+    achieved = (2 * V + C) / (ms * 1e-3)
+    ms32 = fm.time_dslash(lat, "fp32", False, nbuf=4, iters=200)
+    ms16 = fm.time_dslash(lat, "bf16", False, nbuf=4, iters=200)
+    print("bench: dslash fp64 %.2f us (%.0f GB/s), fp32 %.2f us, bf16 %.2f us" % (
+        ms * 1e3, achieved / 1e9, ms32 * 1e3, ms16 * 1e3), file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": wall * 1e3,
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32+f64", "bf16": "bf16+f32+f64"}[args.form],
+        "data": "synthetic (seeded phi ~ N(0,1/beta), complex Gaussian source, reference RNG)",
+        "config": {"workload": "M^dagger M solve, domain-wall operator, L=Lt=%d Ls=%d beta=%g M5=1 imass=1, "
+                               "to FP64 true relative residual %g" % (args.L, args.Ls, args.beta, args.tol),
+                   "form": args.form, "L": args.L, "Ls": args.Ls, "tol": args.tol,
+                   "parallelism": "single GPU"},
+        "solve": {"form": args.form, "iterations": st["iterations"], "reliable_updates": st["reliable_updates"],
+                  "true_resid": st["true_resid"], "device_ms": st["seconds"] * 1e3},
+        "fp64_solve": {"ms": st64["seconds"] * 1e3, "iterations": st64["iterations"],
+                       "true_resid": st64["true_resid"]},
+        "speedup_vs_fp64": st64["seconds"] / max(wall, 1e-12),
+        "dslash_us": {"fp64": ms * 1e3, "fp32": ms32 * 1e3, "bf16": ms16 * 1e3},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_dslash<F64,F64,false,EPI_STORE>", "bytes_per_launch": 2 * V + C},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.L, args.Ls, args.beta, st64["iterations"])
+    else:
+        out["cpu_baseline"] = None
+    lat.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

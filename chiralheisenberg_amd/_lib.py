@@ -1,0 +1,100 @@
+"""ctypes binding of libchs.so (the HIP/gfx950 library).  There is no CPU
+fallback: if the library is missing or no GPU is present, loading fails with
+an error."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libchs.so")
+
+CHS_OK, CHS_EINVAL, CHS_EHIP, CHS_ENOCONV, CHS_ENOMEM = 0, 1, 2, 3, 4
+CHS_FP64, CHS_FP32, CHS_BF16 = 0, 1, 2
+PRECISIONS = {"fp64": CHS_FP64, "fp32": CHS_FP32, "bf16": CHS_BF16}
+BYTES_PER_COMPLEX = {CHS_FP64: 16, CHS_FP32: 8, CHS_BF16: 4}
+MAX_SHIFTS = 32
+
+_lib = None
+
+
+class ChsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("libchs error %d: %s" % (code, msg))
+        self.code = code
+
+
+class NoConvergence(ChsError):
+    pass
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("reliable_updates", ctypes.c_int), ("restarts", ctypes.c_int),
+                ("true_resid", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+class Rng(ctypes.Structure):
+    _fields_ = [("s", ctypes.c_uint64), ("v", ctypes.c_double * 97), ("idum", ctypes.c_int),
+                ("yran", ctypes.c_double), ("yrun", ctypes.c_double)]
+
+
+def _declare(L):
+    c, i, d, l, v, sz = ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_long, ctypes.c_void_p, ctypes.c_size_t
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "chs_ctx_create": ([i, i, i, i, d, i], v),
+        "chs_ctx_destroy": ([v], None),
+        "chs_last_error": ([v], c),
+        "chs_create_error": ([], c),
+        "chs_kferm": ([v], l),
+        "chs_kvol": ([v], l),
+        "chs_stream": ([v], v),
+        "chs_synchronize": ([v], i),
+        "chs_dslash": ([v, v, v, v, d, d, i], i),
+        "chs_dslashd": ([v, v, v, v, d, d, i], i),
+        "chs_qmrherm": ([v, v, d, ip, d, d, i, v, v, i, i, v, v, v], i),
+        "chs_congrad": ([v, v, d, ip, d, d, i, v, v], i),
+        "chs_force": ([v, v, d, d, d, i, v, d, v, v, v, v, i, v, v, ip], i),
+        "chs_hamilton": ([v, v, v, v, v, v, d, i, d, d, i, v, v, d, v, v, i, v, ip], i),
+        "chs_derivs": ([v, v, v, d, i, v], i),
+        "chs_alloc": ([v, sz], v),
+        "chs_free": ([v, v], i),
+        "chs_upload": ([v, v, v, sz], i),
+        "chs_download": ([v, v, v, sz], i),
+        "chs_convert_dev": ([v, v, i, v, i, l], i),
+        "chs_dslash_dev": ([v, v, v, v, d, i, i, i], i),
+        "chs_qmrherm_dev": ([v, v, d, ip, d, i, v, v, i, i, v, v, v], i),
+        "chs_congrad_dev": ([v, v, d, ip, d, i, v, v], i),
+        "chs_solve_mdagm_dev": ([v, v, v, v, d, i, d, i, d, i, ctypes.POINTER(SolveStats)], i),
+        "chs_time_dslash": ([v, i, i, i, i, d, i, ctypes.POINTER(ctypes.c_double)], i),
+        "chs_rng_set": ([ctypes.POINTER(Rng), d], None),
+        "chs_rng_get": ([ctypes.POINTER(Rng)], d),
+        "chs_rano": ([ctypes.POINTER(Rng), i], d),
+        "chs_gauss0": ([ctypes.POINTER(Rng), v, i], None),
+        "chs_gaussp": ([ctypes.POINTER(Rng), v, i], None),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+
+
+def lib():
+    """The loaded libchs.so.  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libchs.so not built: run `python -m chiralheisenberg_amd.build` "
+                              "(or __graft_entry__.build()) first")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def check(ctx_handle, rc):
+    if rc == CHS_OK:
+        return
+    msg = lib().chs_last_error(ctx_handle)
+    msg = msg.decode() if msg else ""
+    if rc == CHS_ENOCONV:
+        raise NoConvergence(rc, msg)
+    raise ChsError(rc, msg)

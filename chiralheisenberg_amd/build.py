@@ -1,0 +1,48 @@
+"""Build recipe of libchs.so (hipcc for gfx950, in-tree) and of the Fortran
+shim module.  Called by __graft_entry__.build(); also runnable as
+`python -m chiralheisenberg_amd.build`."""
+import os
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libchs.so")
+SOURCES = [os.path.join(HERE, "csrc", "chs_api.hip"), os.path.join(HERE, "csrc", "chs_rng.cpp")]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("CHS_OFFLOAD_ARCH", "gfx950")
+
+# Packed FP32 instructions are switched off for all device code: the compiler
+# would otherwise form v_pk_* FP32 operations from float2 arithmetic.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=" + ARCH,
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result", "-Wno-unused-value"]
+
+
+def build_lib(verbose=False):
+    """Compile libchs.so in place; returns its path."""
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC] + HIP_FLAGS + ["-I" + os.path.join(ROOT, "include")] + SOURCES + ["-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + res.stderr[-4000:])
+    shutil.move(tmp, LIB)
+    return LIB
+
+
+def build_fortran_shim(outdir, flang="/opt/rocm/llvm/bin/flang", defines=None):
+    """Compile the iso_c_binding shim (chs_shim.F90) into an object in outdir."""
+    src = os.path.join(HERE, "fortran", "chs_shim.F90")
+    os.makedirs(outdir, exist_ok=True)
+    obj = os.path.join(outdir, "chs_shim.o")
+    cmd = [flang, "-O2", "-fPIC", "-c", src, "-o", obj, "-module-dir", outdir]
+    for k, v in (defines or {}).items():
+        cmd.append("-D%s=%s" % (k, v))
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return obj
+
+
+if __name__ == "__main__":
+    print(build_lib(verbose=True))

@@ -1,0 +1,782 @@
+// chs_api.hip -- host side of libchs: context, device buffers, kernel
+// launch sequences of the solvers and the extern "C" entry points declared in
+// include/chs.h.
+//
+// The reference routines (rhmc/cheisdwf.f) operate on host arrays and COMMON
+// blocks; here every field lives in HBM for the duration of a call (the
+// drop-in entry points upload their inputs once and download their outputs
+// once) or permanently (the _dev entry points).  Solver iterations run as
+// fixed sequences of kernels whose scalars stay on the device; the host
+// enqueues several iterations ahead and polls a halt flag, so no host round
+// trip sits inside an iteration.
+#include "chs_device.h"
+#include "../../include/chs.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace chs;
+
+struct chs_ctx {
+    int L, Lt, Ls, ibound, device;
+    double am3;
+    long kvol, kferm, n5;
+    Lat lat;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // device work space, grown on demand and kept
+    struct Buf { void *p = nullptr; size_t n = 0; };
+    Buf b[16];
+    Buf shifts_p, shifts_x, contrib;
+    double *partial = nullptr, *partial2 = nullptr;
+    int npart = 0;
+    LzState *lz = nullptr;
+    CgState *cg = nullptr;
+    double *dscal = nullptr;      // small device scalar array
+    double *dcoef = nullptr;      // remez coefficients (anum) on the device
+    void *hpinned = nullptr;      // pinned host scratch for polling
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static thread_local std::string g_create_err;
+
+namespace {
+
+enum BufId { B_PHI = 0, B_X, B_R, B_Q0, B_Q1, B_V, B_W, B_CHI, B_F, B_PP, B_P, B_T1, B_T2, B_XS, B_RS, B_PS };
+
+int fail(chs_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail((c), CHS_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+inline unsigned nblocks(long n) {
+    long nb = (n + kBlock - 1) / kBlock;
+    nb = (nb + 7) / 8 * 8;   // multiple of the XCD count for logical_block()
+    return (unsigned)(nb > 0 ? nb : 8);
+}
+
+int ensure(chs_ctx *c, chs_ctx::Buf &b, size_t bytes) {
+    if (b.n >= bytes) return CHS_OK;
+    if (b.p) HIPCHK(c, hipFree(b.p));
+    b.p = nullptr;
+    b.n = 0;
+    HIPCHK(c, hipMalloc(&b.p, bytes));
+    b.n = bytes;
+    return CHS_OK;
+}
+
+template <typename T> T *buf(chs_ctx *c, int id) { return (T *)c->b[id].p; }
+
+// ensure a spinor-sized buffer of `bytes_per_complex` for every listed id
+int ensure_spinors(chs_ctx *c, std::initializer_list<int> ids, size_t bpc = 16) {
+    for (int id : ids) {
+        int rc = ensure(c, c->b[id], (size_t)c->kferm * bpc);
+        if (rc) return rc;
+    }
+    return CHS_OK;
+}
+
+int ensure_partials(chs_ctx *c) {
+    if (c->partial) return CHS_OK;
+    const int np = (int)nblocks(c->kferm);   // the largest grid of any reduction
+    HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * np));
+    HIPCHK(c, hipMalloc(&c->partial2, sizeof(double) * np));
+    c->npart = np;
+    return CHS_OK;
+}
+
+size_t prec_bytes(int prec) { return prec == CHS_FP64 ? 16 : prec == CHS_FP32 ? 8 : 4; }
+
+// ---- launchers --------------------------------------------------------------
+template <class S, class SA, bool DAG, int EPI>
+void dslash_launch(chs_ctx *c, void *out, const void *in, const double *chi, double hpin, int imass,
+                   EpiArgs ea) {
+    hipLaunchKernelGGL((k_dslash<S, SA, DAG, EPI>), dim3(nblocks(c->n5)), dim3(kBlock), 0, c->stream, c->lat,
+                       (typename S::st *)out, (const typename S::st *)in, chi, hpin, imass, ea);
+}
+
+EpiArgs epi(double *partial = nullptr, const double *scal = nullptr, void *a0 = nullptr, void *a1 = nullptr,
+            void *a2 = nullptr, const int *skip = nullptr, int flag = 0) {
+    EpiArgs e;
+    e.partial = partial;
+    e.scal = scal;
+    e.aux0 = a0;
+    e.aux1 = a1;
+    e.aux2 = a2;
+    e.skip = skip;
+    e.flag = flag;
+    return e;
+}
+
+template <bool DAG>
+void dslash_prec(chs_ctx *c, void *out, const void *in, const double *chi, double hpin, int imass, int prec) {
+    if (prec == CHS_FP64) dslash_launch<F64, F64, DAG, EPI_STORE>(c, out, in, chi, hpin, imass, epi());
+    else if (prec == CHS_FP32) dslash_launch<F32, F32, DAG, EPI_STORE>(c, out, in, chi, hpin, imass, epi());
+    else dslash_launch<B16, F32, DAG, EPI_STORE>(c, out, in, chi, hpin, imass, epi());
+}
+
+// |a|^2 (mode 0) / Re<a,b> (mode 1) / sum a^2 over reals (mode 2) into dst (device)
+void reduce_launch(chs_ctx *c, long n, const void *a, const void *b, int mode, double *dst) {
+    long nb = (n + kBlock - 1) / kBlock;
+    if (nb > 2048) nb = 2048;
+    nb = (nb + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_reduce, dim3((unsigned)nb), dim3(kBlock), 0, c->stream, n, (const double2 *)a,
+                       (const double2 *)b, mode, c->partial2);
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBlock), 0, c->stream, (const double *)c->partial2, (int)nb,
+                       dst);
+}
+
+int check_imass(chs_ctx *c, int imass) {
+    // the reference applies no wall term for other values (rhmc/cheisdwf.f:2912,2996); keep that
+    (void)c;
+    (void)imass;
+    return CHS_OK;
+}
+
+int poll_int(chs_ctx *c, const int *dev, int *out) {
+    HIPCHK(c, hipMemcpyAsync(c->hpinned, dev, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = *(int *)c->hpinned;
+    return CHS_OK;
+}
+
+// how many solver iterations to enqueue between polls: about 2 ms of work
+int chunk_for(chs_ctx *c, double bytes_per_iter) {
+    double t_us = bytes_per_iter / 5.0e6;   // ~5 TB/s
+    int ch = (int)(2000.0 / (t_us + 20.0));
+    if (ch < 2) ch = 2;
+    if (ch > 64) ch = 64;
+    (void)c;
+    return ch;
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char *chs_create_error(void) { return g_create_err.c_str(); }
+
+chs_ctx *chs_ctx_create(int L, int Lt, int Ls, int ibound, double am3, int device) {
+    if (L < 2 || Lt < 2 || Ls < 2) {
+        g_create_err = "lattice extents must be >= 2 (L, Lt, Ls)";
+        return nullptr;
+    }
+    long kvol = (long)L * L * Lt;
+    if (8L * Ls * kvol > (1L << 31) - 1) {
+        g_create_err = "lattice too large for 32-bit site indexing";
+        return nullptr;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        g_create_err = "hipSetDevice failed (no GPU?)";
+        return nullptr;
+    }
+    chs_ctx *c = new chs_ctx();
+    c->L = L;
+    c->Lt = Lt;
+    c->Ls = Ls;
+    c->ibound = ibound;
+    c->am3 = am3;
+    c->device = device;
+    c->kvol = kvol;
+    c->n5 = (long)Ls * kvol;
+    c->kferm = 8 * c->n5;
+    c->lat.L = L;
+    c->lat.Lt = Lt;
+    c->lat.Ls = Ls;
+    c->lat.kvol = (int)kvol;
+    c->lat.n5 = c->n5;
+    c->lat.diag = 4.0 - am3;
+    c->lat.apbc = (ibound == -1);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&c->lz, sizeof(LzState)) != hipSuccess || hipMalloc(&c->cg, sizeof(CgState)) != hipSuccess ||
+        hipMalloc(&c->dscal, 64 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->dcoef, 2 * (kMaxShift + 1) * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&c->hpinned, 4096, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        g_create_err = "HIP allocation failed in chs_ctx_create";
+        chs_ctx_destroy(c);
+        return nullptr;
+    }
+    if (ensure_partials(c) != CHS_OK) {
+        g_create_err = c->err;
+        chs_ctx_destroy(c);
+        return nullptr;
+    }
+    return c;
+}
+
+void chs_ctx_destroy(chs_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &b : c->b)
+        if (b.p) hipFree(b.p);
+    if (c->shifts_p.p) hipFree(c->shifts_p.p);
+    if (c->shifts_x.p) hipFree(c->shifts_x.p);
+    if (c->contrib.p) hipFree(c->contrib.p);
+    if (c->partial) hipFree(c->partial);
+    if (c->partial2) hipFree(c->partial2);
+    if (c->lz) hipFree(c->lz);
+    if (c->cg) hipFree(c->cg);
+    if (c->dscal) hipFree(c->dscal);
+    if (c->dcoef) hipFree(c->dcoef);
+    if (c->hpinned) hipHostFree(c->hpinned);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *chs_last_error(const chs_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+long chs_kferm(const chs_ctx *c) { return c->kferm; }
+long chs_kvol(const chs_ctx *c) { return c->kvol; }
+void *chs_stream(chs_ctx *c) { return (void *)c->stream; }
+int chs_synchronize(chs_ctx *c) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHS_OK;
+}
+
+void *chs_alloc(chs_ctx *c, size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        fail(c, CHS_ENOMEM, "hipMalloc(%zu) failed", bytes);
+        return nullptr;
+    }
+    return p;
+}
+int chs_free(chs_ctx *c, void *p) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(p));
+    return CHS_OK;
+}
+int chs_upload(chs_ctx *c, void *dst, const void *src, size_t bytes) {
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHS_OK;
+}
+int chs_download(chs_ctx *c, void *dst, const void *src, size_t bytes) {
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHS_OK;
+}
+
+int chs_convert_dev(chs_ctx *c, void *dst, int dp, const void *src, int sp, long n) {
+    if (dp < 0 || dp > 2 || sp < 0 || sp > 2 || n < 0) return fail(c, CHS_EINVAL, "bad precision or size");
+    dim3 g(nblocks(n)), b(kBlock);
+#define CV(SD, SS)                                                                                           \
+    hipLaunchKernelGGL((k_convert<SD, SS>), g, b, 0, c->stream, n, (typename SD::st *)dst,                  \
+                       (const typename SS::st *)src)
+    if (dp == CHS_FP64 && sp == CHS_FP64) CV(F64, F64);
+    else if (dp == CHS_FP64 && sp == CHS_FP32) CV(F64, F32);
+    else if (dp == CHS_FP64 && sp == CHS_BF16) CV(F64, B16);
+    else if (dp == CHS_FP32 && sp == CHS_FP64) CV(F32, F64);
+    else if (dp == CHS_FP32 && sp == CHS_FP32) CV(F32, F32);
+    else if (dp == CHS_FP32 && sp == CHS_BF16) CV(F32, B16);
+    else if (dp == CHS_BF16 && sp == CHS_FP64) CV(B16, F64);
+    else if (dp == CHS_BF16 && sp == CHS_FP32) CV(B16, F32);
+    else CV(B16, B16);
+#undef CV
+    HIPCHK(c, hipGetLastError());
+    return CHS_OK;
+}
+
+// ---- operators ------------------------------------------------------------------
+int chs_dslash_dev(chs_ctx *c, void *out, const void *in, const double *chi, double hpin, int imass, int dagger,
+                   int prec) {
+    if (prec < 0 || prec > 2) return fail(c, CHS_EINVAL, "precision %d not in {0,1,2}", prec);
+    if (out == in) return fail(c, CHS_EINVAL, "dslash output must not alias its input");
+    if (dagger) dslash_prec<true>(c, out, in, chi, hpin, imass, prec);
+    else dslash_prec<false>(c, out, in, chi, hpin, imass, prec);
+    HIPCHK(c, hipGetLastError());
+    return CHS_OK;
+}
+
+static int dslash_host(chs_ctx *c, double *phi, const double *r, const double *chi, double hpin, int imass,
+                       int dagger) {
+    int rc = ensure_spinors(c, {B_R, B_V});
+    if (rc) return rc;
+    rc = ensure(c, c->b[B_CHI], sizeof(double) * 3 * c->kvol);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->b[B_R].p, r, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->b[B_CHI].p, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    rc = chs_dslash_dev(c, c->b[B_V].p, c->b[B_R].p, buf<double>(c, B_CHI), hpin, imass, dagger, CHS_FP64);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(phi, c->b[B_V].p, 16 * c->kferm, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHS_OK;
+}
+
+int chs_dslash(chs_ctx *c, double *phi, const double *r, const double *chi, double am, double hpin, int imass) {
+    (void)am;
+    return dslash_host(c, phi, r, chi, hpin, imass, 0);
+}
+int chs_dslashd(chs_ctx *c, double *phi, const double *r, const double *chi, double am, double hpin, int imass) {
+    (void)am;
+    return dslash_host(c, phi, r, chi, hpin, imass, 1);
+}
+
+// ---- multi-shift Lanczos ------------------------------------------------------------
+int chs_qmrherm_dev(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass,
+                    const double *anum, const double *aden, int ndiag, int iflag, const double *chi, double *x,
+                    double *dsdchi) {
+    if (ndiag < 1 || ndiag > kMaxShift) return fail(c, CHS_EINVAL, "ndiag=%d outside 1..%d", ndiag, kMaxShift);
+    if (iflag == 1 && !dsdchi) return fail(c, CHS_EINVAL, "iflag=1 needs dsdchi");
+    const long n = c->kferm;
+    int rc = ensure_spinors(c, {B_R, B_Q0, B_Q1, B_V, B_W});
+    if (rc) return rc;
+    if ((rc = ensure(c, c->shifts_p, (size_t)ndiag * n * 16))) return rc;
+    if ((rc = ensure(c, c->shifts_x, (size_t)ndiag * n * 16))) return rc;
+    double2 *r = buf<double2>(c, B_R), *q0 = buf<double2>(c, B_Q0), *q1 = buf<double2>(c, B_Q1);
+    double2 *v = buf<double2>(c, B_V), *w = buf<double2>(c, B_W);
+    double2 *pk = (double2 *)c->shifts_p.p, *xk = (double2 *)c->shifts_x.p;
+
+    // scalar state: aden, ndiag; coefficients anum for the combine / force
+    LzState h;
+    memset(&h, 0, sizeof h);
+    h.ndiag = ndiag;
+    for (int k = 0; k < ndiag; k++) h.aden[k] = aden[k];
+    HIPCHK(c, hipMemcpyAsync(c->lz, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->dcoef, anum, sizeof(double) * (ndiag + 1), hipMemcpyHostToDevice, c->stream));
+
+    // x = a0 Phi; r = Phi; q_{0} = 0; beta = phimod = |Phi|
+    hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, anum[0], (const double2 *)phi,
+                       0.0, (double2 *)x);
+    HIPCHK(c, hipMemcpyAsync(r, phi, 16 * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(q1, 0, 16 * n, c->stream));
+    reduce_launch(c, n, phi, nullptr, 0, c->dscal);
+    hipLaunchKernelGGL(k_lz_init, dim3(1), dim3(kBlock), 0, c->stream, c->lz, (const double *)c->dscal, 1,
+                       std::sqrt((double)n * res * res));
+
+    const int niterc = 5000;
+    const int np1 = (int)nblocks(c->n5), np2 = (int)nblocks(n);
+    const int chunk = chunk_for(c, (12.0 + 4.0 * ndiag) * 16.0 * n);
+    const int *skip = &c->lz->done;
+    int done = 0, iter = 1;
+    while (iter <= niterc && !done) {
+        const int last = std::min(niterc, iter + chunk - 1);
+        for (; iter <= last; iter++) {
+            double2 *qc = (iter & 1) ? q0 : q1, *qp = (iter & 1) ? q1 : q0;
+            dslash_launch<F64, F64, false, EPI_SCALEQ>(c, v, r, chi, hpin, imass,
+                                                      epi(nullptr, &c->lz->beta, qc, nullptr, nullptr, skip));
+            dslash_launch<F64, F64, true, EPI_DOT>(c, w, v, chi, hpin, imass,
+                                                  epi(c->partial, nullptr, qc, nullptr, nullptr, skip));
+            hipLaunchKernelGGL(k_lz_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->lz, (const double *)c->partial,
+                               np1, iter);
+            if (ndiag == 25)
+                hipLaunchKernelGGL(k_lz_update<25>, dim3(np2), dim3(kBlock), 0, c->stream, n, (const double2 *)w,
+                                   (const double2 *)qc, (const double2 *)qp, r, pk, xk, (const LzState *)c->lz,
+                                   ndiag, iter, c->partial2);
+            else if (ndiag == 12)
+                hipLaunchKernelGGL(k_lz_update<12>, dim3(np2), dim3(kBlock), 0, c->stream, n, (const double2 *)w,
+                                   (const double2 *)qc, (const double2 *)qp, r, pk, xk, (const LzState *)c->lz,
+                                   ndiag, iter, c->partial2);
+            else
+                hipLaunchKernelGGL(k_lz_update<0>, dim3(np2), dim3(kBlock), 0, c->stream, n, (const double2 *)w,
+                                   (const double2 *)qc, (const double2 *)qp, r, pk, xk, (const LzState *)c->lz,
+                                   ndiag, iter, c->partial2);
+            hipLaunchKernelGGL(k_lz_beta, dim3(1), dim3(kBlock), 0, c->stream, c->lz, (const double *)c->partial2,
+                               np2, iter);
+        }
+        HIPCHK(c, hipGetLastError());
+        if ((rc = poll_int(c, &c->lz->done, &done))) return rc;
+    }
+    int it = 0;
+    if ((rc = poll_int(c, &c->lz->itercg, &it))) return rc;
+    if (itercg) *itercg = it;
+
+    if (iflag == 0) {
+        hipLaunchKernelGGL(k_lz_combine, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const double2 *)phi,
+                           (const double2 *)xk, ndiag, (const double *)c->dcoef, (double2 *)x);
+    } else {
+        const size_t cb = (size_t)ndiag * 3 * c->kvol * sizeof(double);
+        if ((rc = ensure(c, c->contrib, cb))) return rc;
+        hipLaunchKernelGGL(k_force_shift, dim3(nblocks((long)c->kvol * ndiag)), dim3(kBlock), 0, c->stream, c->lat,
+                           (const double2 *)xk, n, ndiag, (const double *)c->dcoef, chi, hpin, imass,
+                           (double *)c->contrib.p);
+        hipLaunchKernelGGL(k_force_sum, dim3(nblocks(3 * c->kvol)), dim3(kBlock), 0, c->stream, (int)c->kvol,
+                           ndiag, (const double *)c->contrib.p, dsdchi);
+    }
+    HIPCHK(c, hipGetLastError());
+    // the reference returns silently after niterc iterations (rhmc/cheisdwf.f:639)
+    return CHS_OK;
+}
+
+int chs_qmrherm(chs_ctx *c, const double *phi, double res, int *itercg, double am, double hpin, int imass,
+                const double *anum, const double *aden, int ndiag, int iflag, const double *chi, double *x,
+                double *dsdchi) {
+    (void)am;
+    int rc = ensure_spinors(c, {B_PHI, B_X});
+    if (rc) return rc;
+    if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
+    if ((rc = ensure(c, c->b[B_F], 24 * c->kvol))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->b[B_PHI].p, phi, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->b[B_CHI].p, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    if (iflag == 1) {
+        if (!dsdchi) return fail(c, CHS_EINVAL, "iflag=1 needs dsdchi");
+        HIPCHK(c, hipMemcpyAsync(c->b[B_F].p, dsdchi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    }
+    rc = chs_qmrherm_dev(c, buf<double>(c, B_PHI), res, itercg, hpin, imass, anum, aden, ndiag, iflag,
+                         buf<double>(c, B_CHI), buf<double>(c, B_X), buf<double>(c, B_F));
+    if (rc) return rc;
+    if (x) HIPCHK(c, hipMemcpyAsync(x, c->b[B_X].p, 16 * c->kferm, hipMemcpyDeviceToHost, c->stream));
+    if (iflag == 1) HIPCHK(c, hipMemcpyAsync(dsdchi, c->b[B_F].p, 24 * c->kvol, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHS_OK;
+}
+
+// ---- CGNR with the reference recurrences --------------------------------------
+int chs_congrad_dev(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass,
+                    const double *chi, double *x) {
+    const long n = c->kferm;
+    int rc = ensure_spinors(c, {B_P, B_R, B_T1});
+    if (rc) return rc;
+    double2 *p = buf<double2>(c, B_P), *r = buf<double2>(c, B_R), *x1 = buf<double2>(c, B_T1);
+    CgState h;
+    memset(&h, 0, sizeof h);
+    h.alpha = 1.0;
+    h.betad = 1.0;
+    h.resid = (double)n * res * res;
+    h.maxiter = (int)(c->Ls * c->kvol);
+    HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(p, x, 16 * n, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(r, phi, 16 * n, hipMemcpyDeviceToDevice, c->stream));
+    const int np = (int)nblocks(c->n5);
+    const int *skip = &c->cg->halt;
+    const int chunk = chunk_for(c, 11.0 * 16.0 * n);
+    int halt = 0, nx = 1;
+    while (nx <= h.maxiter && !halt) {
+        const int last = std::min(h.maxiter, nx + chunk - 1);
+        for (; nx <= last; nx++) {
+            if (nx == 1) {
+                dslash_launch<F64, F64, false, EPI_STORE>(c, x1, p, chi, hpin, imass,
+                                                         epi(nullptr, nullptr, nullptr, nullptr, nullptr, skip));
+                dslash_launch<F64, F64, true, EPI_CG>(c, nullptr, x1, chi, hpin, imass,
+                                                     epi(c->partial, &c->cg->alpha, p, x, r, skip, 1));
+            } else {
+                dslash_launch<F64, F64, false, EPI_NORM>(c, x1, p, chi, hpin, imass,
+                                                        epi(c->partial, nullptr, nullptr, nullptr, nullptr, skip));
+                hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->cg,
+                                   (const double *)c->partial, np);
+                dslash_launch<F64, F64, true, EPI_CG>(c, nullptr, x1, chi, hpin, imass,
+                                                     epi(c->partial, &c->cg->alpha, p, x, r, skip, 0));
+            }
+            hipLaunchKernelGGL(k_cg_beta_ref, dim3(1), dim3(kBlock), 0, c->stream, c->cg, (const double *)c->partial,
+                               np, nx);
+            hipLaunchKernelGGL((k_cg_pupdate<F64, F64>), dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, p,
+                               (const double2 *)r, (const CgState *)c->cg);
+        }
+        HIPCHK(c, hipGetLastError());
+        if ((rc = poll_int(c, &c->cg->halt, &halt))) return rc;
+    }
+    CgState fin;
+    HIPCHK(c, hipMemcpyAsync(&fin, c->cg, sizeof fin, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (itercg) *itercg = fin.iter;
+    if (!(fin.rnorm2 < fin.resid))
+        return fail(c, CHS_ENOCONV, "# iterations of congrad exceeds niterc (%d)", fin.iter);
+    return CHS_OK;
+}
+
+int chs_congrad(chs_ctx *c, const double *phi, double res, int *itercg, double am, double hpin, int imass,
+                const double *chi, double *x) {
+    (void)am;
+    int rc = ensure_spinors(c, {B_PHI, B_X});
+    if (rc) return rc;
+    if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->b[B_PHI].p, phi, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->b[B_X].p, x, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->b[B_CHI].p, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    int rc2 = chs_congrad_dev(c, buf<double>(c, B_PHI), res, itercg, hpin, imass, buf<double>(c, B_CHI),
+                              buf<double>(c, B_X));
+    if (rc2 != CHS_OK && rc2 != CHS_ENOCONV) return rc2;
+    HIPCHK(c, hipMemcpyAsync(x, c->b[B_X].p, 16 * c->kferm, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return rc2;
+}
+
+// ---- force, derivs, hamilton ------------------------------------------------
+int chs_force(chs_ctx *c, const double *phi, double res1, double am, double hpin, int imass, const double *chi,
+              double beta, const double *anum, const double *aden, const double *bnum, const double *bden, int ndiag,
+              double *dsdchi, double *x, int *its) {
+    (void)am;
+    int rc = ensure_spinors(c, {B_PHI, B_X, B_T2});
+    if (rc) return rc;
+    if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
+    if ((rc = ensure(c, c->b[B_F], 24 * c->kvol))) return rc;
+    double *dphi = buf<double>(c, B_PHI), *dx = buf<double>(c, B_X), *dx2 = buf<double>(c, B_T2);
+    double *dchi = buf<double>(c, B_CHI), *dF = buf<double>(c, B_F);
+    HIPCHK(c, hipMemcpyAsync(dphi, phi, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_force_init, dim3(nblocks(3 * c->kvol)), dim3(kBlock), 0, c->stream, 3 * c->kvol, beta,
+                       (const double *)dchi, dF);
+    int i1 = 0, i2 = 0;
+    // X = r_b(M^dagger M) Phi with the remez2g b-set, then the force from the a-set
+    // (rhmc/cheisdwf.f:500-513)
+    rc = chs_qmrherm_dev(c, dphi, res1, &i1, hpin, imass, bnum, bden, ndiag, 0, dchi, dx, nullptr);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(dx2, dx, 16 * c->kferm, hipMemcpyDeviceToDevice, c->stream));
+    rc = chs_qmrherm_dev(c, dx2, res1, &i2, hpin, imass, anum, aden, ndiag, 1, dchi, dx, dF);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(dsdchi, dF, 24 * c->kvol, hipMemcpyDeviceToHost, c->stream));
+    if (x) HIPCHK(c, hipMemcpyAsync(x, dx, 16 * c->kferm, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (its) {
+        its[0] = i1;
+        its[1] = i2;
+    }
+    return CHS_OK;
+}
+
+int chs_derivs(chs_ctx *c, const double *r, const double *x2, double anum, int imass, double *dsdchi) {
+    int rc = ensure_spinors(c, {B_T1, B_T2});
+    if (rc) return rc;
+    if ((rc = ensure(c, c->b[B_F], 24 * c->kvol))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->b[B_T1].p, r, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->b[B_T2].p, x2, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->b[B_F].p, dsdchi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_derivs, dim3(nblocks(c->kvol)), dim3(kBlock), 0, c->stream, c->lat,
+                       (const double2 *)c->b[B_T1].p, (const double2 *)c->b[B_T2].p, anum, imass,
+                       buf<double>(c, B_F));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(dsdchi, c->b[B_F].p, 24 * c->kvol, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHS_OK;
+}
+
+int chs_hamilton(chs_ctx *c, const double *phi, double *h, double *hg, double *hp, double *s, double res2,
+                 int isweep, double am, double hpin, int imass, const double *chi, const double *pp, double beta,
+                 const double *bnum, const double *bden, int ndiag, double *x, int *itercg) {
+    (void)am;
+    (void)isweep;
+    int rc = ensure_spinors(c, {B_PHI, B_X});
+    if (rc) return rc;
+    if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
+    if ((rc = ensure(c, c->b[B_PP], 24 * c->kvol))) return rc;
+    double *dphi = buf<double>(c, B_PHI), *dx = buf<double>(c, B_X), *dchi = buf<double>(c, B_CHI);
+    double *dpp = buf<double>(c, B_PP);
+    HIPCHK(c, hipMemcpyAsync(dphi, phi, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dpp, pp, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    // sums of p^2 and chi^2 (rhmc/cheisdwf.f:546-559), FP64 on the device
+    reduce_launch(c, 3 * c->kvol, dpp, nullptr, 2, c->dscal + 8);
+    reduce_launch(c, 3 * c->kvol, dchi, nullptr, 2, c->dscal + 9);
+    int it = 0;
+    rc = chs_qmrherm_dev(c, dphi, res2, &it, hpin, imass, bnum, bden, ndiag, 0, dchi, dx, nullptr);
+    if (rc) return rc;
+    reduce_launch(c, c->kferm, dphi, dx, 1, c->dscal + 10);   // Re Phi^dagger X (rhmc/cheisdwf.f:578-580)
+    double sums[3];
+    HIPCHK(c, hipMemcpyAsync(sums, c->dscal + 8, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (x) HIPCHK(c, hipMemcpyAsync(x, dx, 16 * c->kferm, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double vhp = 0.5 * sums[0], vhg = beta * sums[1], vhf = sums[2];
+    *hp = vhp;
+    *hg = vhg;
+    *h = vhg + vhp + vhf;
+    *s = vhg + vhf;
+    if (itercg) *itercg = it;
+    return CHS_OK;
+}
+
+}  // extern "C"
+
+// ---- M^dagger M solves to an FP64 true residual ----------------------------------
+namespace {
+
+// w = M^dagger M x in FP64 using t as scratch
+void mdagm64(chs_ctx *c, double2 *w, const double2 *x, double2 *t, const double *chi, double hpin, int imass) {
+    dslash_launch<F64, F64, false, EPI_STORE>(c, t, x, chi, hpin, imass, epi());
+    dslash_launch<F64, F64, true, EPI_STORE>(c, w, t, chi, hpin, imass, epi());
+}
+
+template <class S, class SA>
+int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, double hpin, int imass, double tol,
+              double delta, int maxiter, chs_solve_stats *stats) {
+    const long n = c->kferm;
+    const size_t bs = sizeof(typename S::st), ba = sizeof(typename SA::st);
+    int rc;
+    if ((rc = ensure_spinors(c, {B_T1, B_T2, B_R}))) return rc;   // FP64 scratch + true residual
+    if ((rc = ensure(c, c->b[B_PS], n * bs))) return rc;         // p (dslash input)
+    if ((rc = ensure(c, c->b[B_V], n * bs))) return rc;          // M p
+    if ((rc = ensure(c, c->b[B_XS], n * ba))) return rc;         // low-precision x accumulator
+    if ((rc = ensure(c, c->b[B_RS], n * ba))) return rc;         // iterated residual
+    double2 *t1 = buf<double2>(c, B_T1), *t2 = buf<double2>(c, B_T2), *r64 = buf<double2>(c, B_R);
+    void *ps = c->b[B_PS].p, *mp = c->b[B_V].p, *xs = c->b[B_XS].p, *rs = c->b[B_RS].p;
+    const double2 *b = (const double2 *)bvec;
+    double2 *x = (double2 *)xvec;
+    const unsigned gn = nblocks(n);
+    const int np = (int)nblocks(c->n5);
+
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    reduce_launch(c, n, b, nullptr, 0, c->dscal);
+    // true residual r64 = b - M^dagger M x, rs = r64
+    mdagm64(c, t2, x, t1, chi, hpin, imass);
+    hipLaunchKernelGGL((k_true_resid<SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, b, (const double2 *)t2, r64,
+                       (typename SA::st *)rs, c->partial2);
+    hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBlock), 0, c->stream, (const double *)c->partial2, (int)gn,
+                       c->dscal + 1);
+    double s2[2];
+    HIPCHK(c, hipMemcpyAsync(s2, c->dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double bnorm2 = s2[0];
+    double rn2 = s2[1];
+    chs_solve_stats st;
+    memset(&st, 0, sizeof st);
+    const double target2 = tol * tol * bnorm2;
+    if (bnorm2 == 0.0) {
+        HIPCHK(c, hipMemsetAsync(x, 0, 16 * n, c->stream));
+        rn2 = 0.0;
+    }
+    int status = CHS_OK;
+    if (rn2 > target2) {
+        // p = rs, xs = 0
+        hipLaunchKernelGGL((k_convert<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, (typename S::st *)ps,
+                           (const typename SA::st *)rs);
+        HIPCHK(c, hipMemsetAsync(xs, 0, n * ba, c->stream));
+        CgState h;
+        memset(&h, 0, sizeof h);
+        h.rnorm2 = rn2;
+        h.maxr2 = rn2;
+        h.delta2 = delta * delta;
+        double stop_fac = 0.5;
+        h.stop2 = stop_fac * target2;
+        h.maxiter = maxiter;
+        HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+        const int *skip = &c->cg->halt;
+        const int chunk = chunk_for(c, 11.0 * bs * n);
+        for (;;) {
+            for (int k = 0; k < chunk; k++) {
+                dslash_launch<S, SA, false, EPI_NORM>(c, mp, ps, chi, hpin, imass,
+                                                     epi(c->partial, nullptr, nullptr, nullptr, nullptr, skip));
+                hipLaunchKernelGGL(k_rcg_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->cg,
+                                   (const double *)c->partial, np);
+                dslash_launch<S, SA, true, EPI_CG>(c, nullptr, mp, chi, hpin, imass,
+                                                  epi(c->partial, &c->cg->alpha, ps, xs, rs, skip, 0));
+                hipLaunchKernelGGL(k_rcg_beta, dim3(1), dim3(kBlock), 0, c->stream, c->cg,
+                                   (const double *)c->partial, np);
+                hipLaunchKernelGGL((k_cg_pupdate<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
+                                   (typename S::st *)ps, (const typename SA::st *)rs, (const CgState *)c->cg);
+            }
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipMemcpyAsync(&h, c->cg, sizeof h, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (!h.hold) continue;
+            // reliable update: x += xs; r64 = b - M^dagger M x; rs = r64
+            hipLaunchKernelGGL((k_accum_x<SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, x,
+                               (typename SA::st *)xs);
+            mdagm64(c, t2, x, t1, chi, hpin, imass);
+            hipLaunchKernelGGL((k_true_resid<SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, b, (const double2 *)t2,
+                               r64, (typename SA::st *)rs, c->partial2);
+            hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBlock), 0, c->stream, (const double *)c->partial2,
+                               (int)gn, c->dscal + 1);
+            HIPCHK(c, hipMemcpyAsync(&rn2, c->dscal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            st.reliable_updates++;
+            if (rn2 <= target2) break;
+            if (!(rn2 == rn2)) {
+                status = fail(c, CHS_ENOCONV, "non-finite residual after %d iterations", h.iter);
+                break;
+            }
+            if (h.iter >= maxiter) {
+                status = fail(c, CHS_ENOCONV, "no convergence to %g in %d iterations (true resid %g)", tol, h.iter,
+                              std::sqrt(rn2 / bnorm2));
+                break;
+            }
+            // the iterated residual had reached the stopping region but the
+            // true one had not: tighten the trigger
+            if (h.rn_hold < h.stop2) {
+                stop_fac *= 0.25;
+                h.stop2 = stop_fac * target2;
+            }
+            // continue CG from the true residual with the search direction kept
+            h.beta = rn2 / h.rnorm2;
+            h.rnorm2 = rn2;
+            h.maxr2 = rn2;
+            h.hold = 0;
+            h.halt = 0;
+            HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+            hipLaunchKernelGGL((k_cg_pupdate<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
+                               (typename S::st *)ps, (const typename SA::st *)rs, (const CgState *)c->cg);
+        }
+        st.iterations = h.iter;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    st.seconds = ms * 1e-3;
+    st.true_resid = bnorm2 > 0 ? std::sqrt(rn2 / bnorm2) : 0.0;
+    if (stats) *stats = st;
+    return status;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chs_solve_mdagm_dev(chs_ctx *c, const double *b, double *x, const double *chi, double hpin, int imass,
+                        double tol, int prec, double delta, int maxiter, chs_solve_stats *stats) {
+    if (!(tol > 0)) return fail(c, CHS_EINVAL, "tol must be > 0");
+    if (maxiter <= 0) maxiter = 100000;
+    if (prec == CHS_FP64) {
+        if (delta <= 0) delta = 1e-30;   // plain CG: true residual only at the end
+        return rcg_solve<F64, F64>(c, b, x, chi, hpin, imass, tol, delta, maxiter, stats);
+    }
+    if (prec == CHS_FP32) {
+        if (delta <= 0) delta = 1e-3;
+        return rcg_solve<F32, F32>(c, b, x, chi, hpin, imass, tol, delta, maxiter, stats);
+    }
+    if (prec == CHS_BF16) {
+        if (delta <= 0) delta = 0.1;
+        return rcg_solve<B16, F32>(c, b, x, chi, hpin, imass, tol, delta, maxiter, stats);
+    }
+    return fail(c, CHS_EINVAL, "precision %d not in {0,1,2}", prec);
+}
+
+// ---- timing helper for the roofline figure ------------------------------------
+int chs_time_dslash(chs_ctx *c, int prec, int dagger, int nbuf, int iters, double hpin, int imass, double *avg_ms) {
+    if (prec < 0 || prec > 2 || nbuf < 1 || iters < 1) return fail(c, CHS_EINVAL, "bad arguments");
+    const size_t bytes = (size_t)c->kferm * prec_bytes(prec);
+    std::vector<void *> ins(nbuf), outs(nbuf);
+    double *chi = nullptr;
+    HIPCHK(c, hipMalloc(&chi, 24 * c->kvol));
+    HIPCHK(c, hipMemsetAsync(chi, 0, 24 * c->kvol, c->stream));
+    for (int i = 0; i < nbuf; i++) {
+        HIPCHK(c, hipMalloc(&ins[i], bytes));
+        HIPCHK(c, hipMalloc(&outs[i], bytes));
+        HIPCHK(c, hipMemsetAsync(ins[i], 0, bytes, c->stream));
+    }
+    for (int i = 0; i < nbuf; i++) chs_dslash_dev(c, outs[i], ins[i], chi, hpin, imass, dagger, prec);   // warm-up
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < iters; i++) chs_dslash_dev(c, outs[i % nbuf], ins[i % nbuf], chi, hpin, imass, dagger, prec);
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = ms / iters;
+    for (int i = 0; i < nbuf; i++) {
+        hipFree(ins[i]);
+        hipFree(outs[i]);
+    }
+    hipFree(chi);
+    return CHS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,274 @@
+"""Host-side mirror of the reference's fermion routines (rhmc/cheisdwf.f),
+running on the MI355X through libchs.
+
+The routine names, argument meanings and order follow the reference:
+  dslash(Phi,R,chi,am,hpin,imass)                          rhmc/cheisdwf.f:2634
+  dslashd(Phi,R,chi,am,hpin,imass)                         rhmc/cheisdwf.f:3071
+  qmrherm(Phi,res,itercg,am,hpin,imass,anum,aden,ndiag,iflag)  rhmc/cheisdwf.f:599
+  congrad(Phi,res,itercg,am,hpin,imass)                    rhmc/cheisdwf.f:1064
+  force(Phi,res1,am,hpin,imass)                            rhmc/cheisdwf.f:470
+  hamilton(Phi,h,hg,hp,s,res2,isweep,am,hpin,imass)        rhmc/cheisdwf.f:522
+  derivs(R,X2,anum,imass)                                  rhmc/cheisdwf.f:747
+The reference's COMMON inputs (/trial/ chit and pp, /para/ beta, the Remez
+blocks) become explicit arguments and its outputs (/vector/, /sforce/)
+return values.  Arrays are numpy complex128 of shape (2, 4, Lt, L, L, Ls) in C
+order, i.e. the reference's Phi(Ls, L, L, Lt, 4, 2); chi, pp and dSdchi are
+float64 (3, kvol).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, CHS_FP64, CHS_FP32, CHS_BF16, PRECISIONS, BYTES_PER_COMPLEX
+
+_vp = ctypes.c_void_p
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp)
+
+
+class Lattice:
+    """One lattice geometry and one HIP stream on one GPU (a libchs context).
+    Replaces the compile-time parameter(ksize, ksizet, kthird) and /para/ am3,
+    ibound of the reference (rhmc/cheisdwf.f:31,40,85)."""
+
+    def __init__(self, L, Ls, Lt=None, am3=1.0, ibound=-1, device=0):
+        self.L, self.Ls = int(L), int(Ls)
+        self.Lt = int(Lt if Lt is not None else L)
+        self.am3, self.ibound = float(am3), int(ibound)
+        h = lib().chs_ctx_create(self.L, self.Lt, self.Ls, self.ibound, self.am3, int(device))
+        if not h:
+            raise _lib.ChsError(_lib.CHS_EINVAL, lib().chs_create_error().decode())
+        self.h = h
+        self.kvol = self.L * self.L * self.Lt
+        self.kferm = 8 * self.Ls * self.kvol
+
+    @property
+    def shape(self):
+        return (2, 4, self.Lt, self.L, self.L, self.Ls)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().chs_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self):
+        check(self.h, lib().chs_synchronize(self.h))
+
+    def _spinor(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        if a.size != self.kferm:
+            raise ValueError("spinor has %d elements, lattice needs %d" % (a.size, self.kferm))
+        return a
+
+    def _field(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != 3 * self.kvol:
+            raise ValueError("field has %d elements, lattice needs %d" % (a.size, 3 * self.kvol))
+        return a
+
+
+# ---- operators --------------------------------------------------------------
+def dslash(lat, R, chi, am=0.0, hpin=0.0, imass=1):
+    """Phi = M R (rhmc/cheisdwf.f:2634-3069)."""
+    R = lat._spinor(R)
+    phi = np.empty_like(R)
+    check(lat.h, lib().chs_dslash(lat.h, _p(phi), _p(R), _p(lat._field(chi)), am, hpin, imass))
+    return phi.reshape(lat.shape)
+
+
+def dslashd(lat, R, chi, am=0.0, hpin=0.0, imass=1):
+    """Phi = M^dagger R (rhmc/cheisdwf.f:3071-3501)."""
+    R = lat._spinor(R)
+    phi = np.empty_like(R)
+    check(lat.h, lib().chs_dslashd(lat.h, _p(phi), _p(R), _p(lat._field(chi)), am, hpin, imass))
+    return phi.reshape(lat.shape)
+
+
+def qmrherm(lat, Phi, res, anum, aden, chi, am=0.0, hpin=0.0, imass=1, iflag=0, dSdchi=None):
+    """Multi-shift Lanczos solve of (M^dagger M + aden_k) x_k = Phi
+    (rhmc/cheisdwf.f:599-745).  Returns (x, itercg): x is /vector/ (a0 Phi +
+    sum a_k x_k for iflag=0, a0 Phi for iflag=1).  For iflag=1 the force
+    contributions are added to dSdchi in place."""
+    Phi = lat._spinor(Phi)
+    anum = np.ascontiguousarray(anum, dtype=np.float64)
+    aden = np.ascontiguousarray(aden, dtype=np.float64)
+    if len(anum) != len(aden) + 1:
+        raise ValueError("anum needs ndiag+1 entries")
+    x = np.empty_like(Phi)
+    it = ctypes.c_int(0)
+    F = None
+    if iflag == 1:
+        if dSdchi is None or dSdchi.dtype != np.float64 or not dSdchi.flags.c_contiguous:
+            raise ValueError("iflag=1 needs a C-contiguous float64 dSdchi to accumulate into")
+        F = dSdchi
+    check(lat.h, lib().chs_qmrherm(lat.h, _p(Phi), res, ctypes.byref(it), am, hpin, imass, _p(anum), _p(aden),
+                                   len(aden), iflag, _p(lat._field(chi)), _p(x), _p(F) if F is not None else None))
+    return x.reshape(lat.shape), it.value
+
+
+def congrad(lat, Phi, res, chi, x0=None, am=0.0, hpin=0.0, imass=1):
+    """CGNR for M^dagger M x = Phi from the initial guess x0
+    (rhmc/cheisdwf.f:1064-1157).  Returns (x, itercg); raises NoConvergence
+    (carrying no result) when the reference would print 'iterations of
+    congrad exceeds niterc'."""
+    Phi = lat._spinor(Phi)
+    x = np.zeros_like(Phi) if x0 is None else np.array(lat._spinor(x0), copy=True)
+    it = ctypes.c_int(0)
+    check(lat.h, lib().chs_congrad(lat.h, _p(Phi), res, ctypes.byref(it), am, hpin, imass, _p(lat._field(chi)),
+                                   _p(x)))
+    return x.reshape(lat.shape), it.value
+
+
+def force(lat, Phi, res1, chi, beta, remez2g, am=0.0, hpin=0.0, imass=1):
+    """dSdchi = 2 beta chi + pseudofermion force (rhmc/cheisdwf.f:470-518).
+    remez2g = (anum, aden, bnum, bden).  Returns (dSdchi, x, (it_guidance,
+    it_force))."""
+    Phi = lat._spinor(Phi)
+    a, ad, b, bd = (np.ascontiguousarray(v, dtype=np.float64) for v in remez2g)
+    F = np.empty((3, lat.kvol))
+    x = np.empty_like(Phi)
+    its = (ctypes.c_int * 2)()
+    check(lat.h, lib().chs_force(lat.h, _p(Phi), res1, am, hpin, imass, _p(lat._field(chi)), beta, _p(a), _p(ad),
+                                 _p(b), _p(bd), len(ad), _p(F), _p(x), its))
+    return F, x.reshape(lat.shape), (its[0], its[1])
+
+
+def hamilton(lat, Phi, res2, chi, pp, beta, remez2, am=0.0, hpin=0.0, imass=1, isweep=1):
+    """H = beta sum chi^2 + 1/2 sum p^2 + Re Phi^dagger r_b(M^dagger M) Phi
+    (rhmc/cheisdwf.f:522-589).  Returns ((h, hg, hp, s), x, itercg)."""
+    Phi = lat._spinor(Phi)
+    _, _, b, bd = (np.ascontiguousarray(v, dtype=np.float64) for v in remez2)
+    out = [ctypes.c_double() for _ in range(4)]
+    x = np.empty_like(Phi)
+    it = ctypes.c_int(0)
+    check(lat.h, lib().chs_hamilton(lat.h, _p(Phi), *[ctypes.byref(o) for o in out], res2, isweep, am, hpin, imass,
+                                    _p(lat._field(chi)), _p(lat._field(pp)), beta, _p(b), _p(bd), len(bd), _p(x),
+                                    ctypes.byref(it)))
+    return tuple(o.value for o in out), x.reshape(lat.shape), it.value
+
+
+def derivs(lat, R, X2, anum, imass, dSdchi):
+    """dSdchi += wall bilinears of R and X2 = M R (rhmc/cheisdwf.f:747-849)."""
+    if dSdchi.dtype != np.float64 or not dSdchi.flags.c_contiguous or dSdchi.size != 3 * lat.kvol:
+        raise ValueError("dSdchi must be C-contiguous float64 (3, kvol)")
+    check(lat.h, lib().chs_derivs(lat.h, _p(lat._spinor(R)), _p(lat._spinor(X2)), float(anum), imass,
+                                  _p(dSdchi)))
+    return dSdchi
+
+
+# ---- device-resident data ----------------------------------------------------
+class DeviceArray:
+    """A buffer in HBM owned by the library (hipMalloc on the context's GPU)."""
+
+    def __init__(self, lat, nbytes):
+        self.lat, self.nbytes = lat, int(nbytes)
+        self.ptr = lib().chs_alloc(lat.h, self.nbytes)
+        if not self.ptr:
+            check(lat.h, _lib.CHS_ENOMEM)
+
+    @classmethod
+    def from_numpy(cls, lat, a):
+        a = np.ascontiguousarray(a)
+        d = cls(lat, a.nbytes)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes == self.nbytes
+        check(self.lat.h, lib().chs_upload(self.lat.h, self.ptr, _p(a), self.nbytes))
+
+    def download(self, dtype, shape=None):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        check(self.lat.h, lib().chs_download(self.lat.h, _p(out), self.ptr, self.nbytes))
+        return out.reshape(shape) if shape is not None else out
+
+    def free(self):
+        if self.ptr:
+            lib().chs_free(self.lat.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.lat.h:
+                self.free()
+        except Exception:
+            pass
+
+
+def spinor_dev(lat, a=None, precision="fp64"):
+    """Device spinor of the given storage precision, optionally filled from a
+    complex128 host array (converted on the device)."""
+    prec = PRECISIONS[precision]
+    d = DeviceArray(lat, lat.kferm * BYTES_PER_COMPLEX[prec])
+    if a is not None:
+        a = lat._spinor(a)
+        if prec == CHS_FP64:
+            d.upload(a)
+        else:
+            tmp = DeviceArray.from_numpy(lat, a)
+            check(lat.h, lib().chs_convert_dev(lat.h, d.ptr, prec, tmp.ptr, CHS_FP64, lat.kferm))
+            lat.synchronize()
+            tmp.free()
+    d.prec = prec
+    return d
+
+
+def to_host(lat, d):
+    """complex128 host copy of a device spinor of any storage precision."""
+    prec = getattr(d, "prec", CHS_FP64)
+    if prec == CHS_FP64:
+        return d.download(np.complex128, lat.shape)
+    tmp = DeviceArray(lat, lat.kferm * 16)
+    check(lat.h, lib().chs_convert_dev(lat.h, tmp.ptr, CHS_FP64, d.ptr, prec, lat.kferm))
+    out = tmp.download(np.complex128, lat.shape)
+    tmp.free()
+    return out
+
+
+def dslash_dev(lat, out, inp, chi_dev, hpin=0.0, imass=1, dagger=False):
+    prec = getattr(inp, "prec", CHS_FP64)
+    check(lat.h, lib().chs_dslash_dev(lat.h, out.ptr, inp.ptr, chi_dev.ptr, hpin, imass, int(bool(dagger)), prec))
+
+
+def solve_mdagm(lat, b_dev, x_dev, chi_dev, tol=1e-10, precision="fp64", hpin=0.0, imass=1, delta=0.0,
+                maxiter=0):
+    """M^dagger M x = b on device arrays to FP64 true relative residual tol.
+    Returns a dict of solve statistics."""
+    st = _lib.SolveStats()
+    check(lat.h, lib().chs_solve_mdagm_dev(lat.h, b_dev.ptr, x_dev.ptr, chi_dev.ptr, hpin, imass, tol,
+                                           PRECISIONS[precision], delta, maxiter, ctypes.byref(st)))
+    return {f: getattr(st, f) for f, _ in _lib.SolveStats._fields_}
+
+
+def qmrherm_dev(lat, phi_dev, res, anum, aden, chi_dev, x_dev, hpin=0.0, imass=1, iflag=0, F_dev=None):
+    anum = np.ascontiguousarray(anum, dtype=np.float64)
+    aden = np.ascontiguousarray(aden, dtype=np.float64)
+    it = ctypes.c_int(0)
+    check(lat.h, lib().chs_qmrherm_dev(lat.h, phi_dev.ptr, res, ctypes.byref(it), hpin, imass, _p(anum), _p(aden),
+                                       len(aden), iflag, chi_dev.ptr, x_dev.ptr,
+                                       F_dev.ptr if F_dev is not None else None))
+    return it.value
+
+
+def time_dslash(lat, precision="fp64", dagger=False, nbuf=4, iters=100, hpin=0.0, imass=1):
+    """Average ms per dslash launch, cycling nbuf buffer pairs (HIP events)."""
+    ms = ctypes.c_double()
+    check(lat.h, lib().chs_time_dslash(lat.h, PRECISIONS[precision], int(bool(dagger)), nbuf, iters, hpin, imass,
+                                       ctypes.byref(ms)))
+    return ms.value

@@ -1,0 +1,196 @@
+/*
+ * chs.h -- C-ABI of libchs, the MI355X (gfx950) implementation of the
+ * fermion hot path of the 2+1D chiral Heisenberg RHMC (reference:
+ * rhmc/cheisdwf.f of j-ostmeyer/chiralHeisenberg).
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Every field is in the reference layout:
+ *    spinors are complex*16 Phi(Ls, L, L, Lt, 4, 2) column-major with
+ *    interleaved (re, im), i.e. element offset s + Ls*(site + kvol*(d + 4 f)),
+ *    site = ix + L*iy + L*L*it (rhmc/cheisdwf.f:50,54,2644); auxiliary field
+ *    chi(kvol, 3) real*8; momenta and forces (kvol, 3) real*8.
+ *  - Entry points whose name has no _dev suffix take HOST arrays exactly as
+ *    the reference routines do (the drop-in boundary); _dev entry points take
+ *    DEVICE pointers into memory that stays resident in HBM.
+ *  - Numerics follow the reference compiled with -fdefault-real-8
+ *    (rhmc/Makefile:21): every real is FP64.  To reproduce the shipped
+ *    REAL*4 rounding of chi inside dslash (rhmc/cheisdwf.f:2917), pass chi
+ *    already rounded to FP32.
+ *  - The reference routines return no status; here every entry point returns
+ *    0 on success and a nonzero CHS_E* code otherwise, and chs_last_error()
+ *    gives the message.  The Fortran shim keeps the reference signatures and
+ *    checks the status itself.
+ *  - One context = one lattice geometry + one HIP stream (default priority).
+ *    Entry points are not reentrant per context; distinct contexts may be
+ *    used from distinct host threads.
+ *  - am (the bare mass) is accepted for signature parity and has no effect,
+ *    as in the reference (rhmc/cheisdwf.f:26).
+ */
+#ifndef CHS_H
+#define CHS_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chs_ctx chs_ctx;
+
+enum {
+    CHS_OK = 0,
+    CHS_EINVAL = 1,     /* bad argument (size, imass, precision, ndiag, ...) */
+    CHS_EHIP = 2,       /* HIP runtime error */
+    CHS_ENOCONV = 3,    /* solver hit its iteration cap (result still written) */
+    CHS_ENOMEM = 4
+};
+
+/* storage precision of a device spinor */
+enum { CHS_FP64 = 0, CHS_FP32 = 1, CHS_BF16 = 2 };
+
+/* Upper bound on the number of rational shifts in one solve. */
+#define CHS_MAX_SHIFTS 32
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the compile-time parameter(ksize, ksizet, kthird) and the COMMON
+ * /para/ am3, ibound (rhmc/cheisdwf.f:31,40,85).  Selects device `device`. */
+chs_ctx *chs_ctx_create(int L, int Lt, int Ls, int ibound, double am3, int device);
+void chs_ctx_destroy(chs_ctx *ctx);
+const char *chs_last_error(const chs_ctx *ctx);
+/* message for a failed chs_ctx_create (no context exists then) */
+const char *chs_create_error(void);
+/* kvol = L*L*Lt, kferm = 8*Ls*kvol */
+long chs_kferm(const chs_ctx *ctx);
+long chs_kvol(const chs_ctx *ctx);
+/* the context's HIP stream, as an opaque handle (hipStream_t) */
+void *chs_stream(chs_ctx *ctx);
+int chs_synchronize(chs_ctx *ctx);
+
+/* ---- operators (host arrays) -------------------------------------------- */
+/* Phi = M R.  Replaces dslash(Phi,R,chi,am,hpin,imass), rhmc/cheisdwf.f:2634. */
+int chs_dslash(chs_ctx *ctx, double *phi, const double *r, const double *chi,
+               double am, double hpin, int imass);
+/* Phi = M^dagger R.  Replaces dslashd, rhmc/cheisdwf.f:3071. */
+int chs_dslashd(chs_ctx *ctx, double *phi, const double *r, const double *chi,
+                double am, double hpin, int imass);
+
+/* ---- solvers (host arrays) ---------------------------------------------- */
+/* Multi-shift Lanczos on (M^dagger M + aden_k) x_k = Phi.
+ * Replaces qmrherm(Phi,res,itercg,am,hpin,imass,anum,aden,ndiag,iflag),
+ * rhmc/cheisdwf.f:599.  The COMMON inputs become arguments: chi = /trial/
+ * chit; x = /vector/ x (output: a0 Phi + sum a_k x_k for iflag=0, a0 Phi for
+ * iflag=1); dsdchi = /sforce/ dSdchi (accumulated for iflag=1, may be NULL
+ * for iflag=0).  anum has ndiag+1 entries, aden ndiag. */
+int chs_qmrherm(chs_ctx *ctx, const double *phi, double res, int *itercg,
+                double am, double hpin, int imass, const double *anum,
+                const double *aden, int ndiag, int iflag, const double *chi,
+                double *x, double *dsdchi);
+
+/* CGNR on M^dagger M x = Phi with x the initial guess on entry and the
+ * solution on exit (/vector/).  Replaces congrad(Phi,res,itercg,am,hpin,
+ * imass), rhmc/cheisdwf.f:1064.  Stops when |r|^2 < kferm res^2 or after
+ * Ls*kvol iterations (then returns CHS_ENOCONV). */
+int chs_congrad(chs_ctx *ctx, const double *phi, double res, int *itercg,
+                double am, double hpin, int imass, const double *chi,
+                double *x);
+
+/* dSdchi = 2 beta chi + pseudofermion force with the remez2g set.
+ * Replaces force(Phi,res1,am,hpin,imass), rhmc/cheisdwf.f:470.  beta =
+ * /para/ beta; anum..bden = /remez2g/; dsdchi = /sforce/ (output); x, if not
+ * NULL, receives /vector/ as the reference leaves it; its, if not NULL,
+ * receives the guidance and force iteration counts. */
+int chs_force(chs_ctx *ctx, const double *phi, double res1, double am,
+              double hpin, int imass, const double *chi, double beta,
+              const double *anum, const double *aden, const double *bnum,
+              const double *bden, int ndiag, double *dsdchi, double *x,
+              int *its);
+
+/* H = beta sum chi^2 + 1/2 sum p^2 + Re Phi^dagger (M^dagger M)^{-1/2} Phi.
+ * Replaces hamilton(Phi,h,hg,hp,s,res2,isweep,am,hpin,imass),
+ * rhmc/cheisdwf.f:522, with /trial/ chit, pp and /remez2/ b-set explicit.
+ * All global sums are FP64 on the device.  x, if not NULL, receives
+ * /vector/; itercg, if not NULL, the iteration count. */
+int chs_hamilton(chs_ctx *ctx, const double *phi, double *h, double *hg,
+                 double *hp, double *s, double res2, int isweep, double am,
+                 double hpin, int imass, const double *chi, const double *pp,
+                 double beta, const double *bnum, const double *bden,
+                 int ndiag, double *x, int *itercg);
+
+/* dSdchi += 2 a wall bilinears of r and x2 = M r.
+ * Replaces derivs(R,X2,anum,imass), rhmc/cheisdwf.f:747. */
+int chs_derivs(chs_ctx *ctx, const double *r, const double *x2, double anum,
+               int imass, double *dsdchi);
+
+/* ---- device-resident interface ------------------------------------------ */
+/* Device buffers allocated by the library (hipMalloc), freed with chs_free. */
+void *chs_alloc(chs_ctx *ctx, size_t bytes);
+int chs_free(chs_ctx *ctx, void *p);
+int chs_upload(chs_ctx *ctx, void *dst_dev, const void *src_host, size_t bytes);
+int chs_download(chs_ctx *ctx, void *dst_host, const void *src_dev, size_t bytes);
+/* Convert a device FP64 spinor to another storage precision and back. */
+int chs_convert_dev(chs_ctx *ctx, void *dst, int dst_prec, const void *src,
+                    int src_prec, long ncomplex);
+
+/* out = M in (dagger=0) or M^dagger in (dagger=1) on device spinors of
+ * storage precision prec (FP64: double2, FP32: float2, BF16: bf16 re,im
+ * pair).  chi is a device real*8 (kvol,3) array. */
+int chs_dslash_dev(chs_ctx *ctx, void *out, const void *in, const double *chi,
+                   double hpin, int imass, int dagger, int prec);
+
+/* Multi-shift Lanczos on device arrays (same recurrences and stopping rule
+ * as chs_qmrherm).  phi, x: device FP64 spinors; dsdchi: device (kvol,3)
+ * FP64, accumulated for iflag=1. */
+int chs_qmrherm_dev(chs_ctx *ctx, const double *phi, double res, int *itercg,
+                    double hpin, int imass, const double *anum,
+                    const double *aden, int ndiag, int iflag,
+                    const double *chi, double *x, double *dsdchi);
+
+/* CGNR with the reference's recurrences on device arrays. */
+int chs_congrad_dev(chs_ctx *ctx, const double *phi, double res, int *itercg,
+                    double hpin, int imass, const double *chi, double *x);
+
+/* Statistics of one M^dagger M solve. */
+typedef struct {
+    int iterations;        /* total inner (Krylov) iterations */
+    int reliable_updates;  /* FP64 true-residual recomputations */
+    int restarts;          /* fallbacks / restarts taken */
+    double true_resid;     /* final |b - M^dagger M x| / |b| in FP64 */
+    double seconds;        /* wall time of the solve on the device stream */
+} chs_solve_stats;
+
+/* Solve M^dagger M x = b on device arrays to the FP64 true relative
+ * residual tol.  prec selects the storage precision of the inner CG
+ * (CHS_FP64: plain FP64 CG; CHS_FP32 / CHS_BF16: reliable-update CG with
+ * FP64 true residuals every time the inner residual has dropped by delta,
+ * delta <= 0 picks a default).  x holds the initial guess on entry. */
+int chs_solve_mdagm_dev(chs_ctx *ctx, const double *b, double *x,
+                        const double *chi, double hpin, int imass, double tol,
+                        int prec, double delta, int maxiter,
+                        chs_solve_stats *stats);
+
+/* Average duration in milliseconds of `iters` back-to-back dslash launches
+ * cycling through nbuf (in, out) device buffer pairs, measured with HIP
+ * events on the context stream.  Used for the HBM roofline figure. */
+int chs_time_dslash(chs_ctx *ctx, int prec, int dagger, int nbuf, int iters,
+                    double hpin, int imass, double *avg_ms);
+
+/* ---- random numbers (rhmc/cheisdwf.f:1678-1801) -------------------------- */
+/* Host-side generator state: 48-bit LCG + 97-entry shuffle shared by the
+ * yran and yrun pointer streams, FP64 uniforms. */
+typedef struct {
+    uint64_t s;
+    double v[97];
+    int idum;
+    double yran, yrun;
+} chs_rng;
+void chs_rng_set(chs_rng *r, double seed);      /* rranset */
+double chs_rng_get(const chs_rng *r);            /* rranget */
+double chs_rano(chs_rng *r, int stream);         /* 0: yran, 1: yrun */
+/* gauss0 (E|z|^2 = 1) and gaussp (N(0,1)): ps(kvol, 2) */
+void chs_gauss0(chs_rng *r, double *ps, int kvol);
+void chs_gaussp(chs_rng *r, double *ps, int kvol);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

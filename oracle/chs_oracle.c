@@ -390,6 +390,34 @@ int orc_congrad(const orc_lat *g, const double *phi, double res,
     return itercg;
 }
 
+/* Exactly niter CGNR iterations of the same recurrences from x = 0 (used to
+ * time a bounded sample of a solve). */
+void orc_cg_iterations(const orc_lat *g, const double *phi, const double *chi,
+                       double hpin, int imass, int niter, double *x) {
+    const long n = kferm_of(g);
+    double *x1 = (double *)malloc(sizeof(double) * 2 * n);
+    double *x2 = (double *)malloc(sizeof(double) * 2 * n);
+    double *p = (double *)malloc(sizeof(double) * 2 * n);
+    double *r = (double *)malloc(sizeof(double) * 2 * n);
+    memset(x, 0, sizeof(double) * 2 * n);
+    memcpy(r, phi, sizeof(double) * 2 * n);
+    memcpy(p, phi, sizeof(double) * 2 * n);
+    double rn = orc_norm2(n, r);
+    for (int it = 0; it < niter; it++) {
+        orc_dslash(g, x1, p, chi, hpin, imass, 0);
+        double alpha = rn / orc_norm2(n, x1);
+        orc_dslash(g, x2, x1, chi, hpin, imass, 1);
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < 2 * n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * x2[i]; }
+        double rn2 = orc_norm2(n, r);
+        double beta = rn2 / rn;
+        rn = rn2;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < 2 * n; i++) p[i] = r[i] + beta * p[i];
+    }
+    free(x1); free(x2); free(p); free(r);
+}
+
 /* ---- force and Hamiltonian ---------------------------------------------- */
 void orc_force(const orc_lat *g, const double *phi, double res1,
                const double *chi, double beta,

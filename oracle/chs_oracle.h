@@ -55,6 +55,10 @@ int orc_qmrherm_shifts(const orc_lat *g, const double *phi, double res,
 int orc_congrad(const orc_lat *g, const double *phi, double res,
                 const double *chi, double hpin, int imass, double *x);
 
+/* Exactly niter CGNR iterations from x = 0 (timing sample). */
+void orc_cg_iterations(const orc_lat *g, const double *phi, const double *chi,
+                       double hpin, int imass, int niter, double *x);
+
 /* force (rhmc/cheisdwf.f:470-518): dSdchi = 2 beta chi + fermion force.
  * x receives the reference's /vector/ contents on exit.  its[0..1] receive
  * the guidance and force iteration counts. */

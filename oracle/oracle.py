@@ -64,6 +64,7 @@ def lib():
         _LIB.orc_qmrherm_shifts.restype = i
         _LIB.orc_congrad.argtypes = [L, p, d, p, d, i, p]
         _LIB.orc_congrad.restype = i
+        _LIB.orc_cg_iterations.argtypes = [L, p, p, d, i, i, p]
         _LIB.orc_force.argtypes = [L, p, d, p, d, p, p, p, p, i, d, i, p, p, p]
         _LIB.orc_hamilton.argtypes = [L, p, d, p, p, d, p, p, i, d, i, p, p]
         _LIB.orc_hamilton.restype = i
@@ -153,6 +154,14 @@ def congrad(lat, phi, res, chi, x0=None, hpin=0.0, imass=1):
     chi = np.ascontiguousarray(chi, dtype=np.float64)
     it = lib().orc_congrad(ctypes.byref(lat.c), _ptr(phi), float(res), _ptr(chi), float(hpin), imass, _ptr(x))
     return x, it
+
+
+def cg_iterations(lat, phi, chi, niter, hpin=0.0, imass=1):
+    phi = _c128(phi, lat)
+    x = np.empty_like(phi)
+    lib().orc_cg_iterations(ctypes.byref(lat.c), _ptr(phi), _ptr(np.ascontiguousarray(chi, dtype=np.float64)),
+                            float(hpin), imass, int(niter), _ptr(x))
+    return x
 
 
 def force(lat, phi, res1, chi, beta, remez2g, hpin=0.0, imass=1):

@@ -1,0 +1,86 @@
+"""GPU parity of dslash/dslashd (rhmc/cheisdwf.f:2634-3501) against the CPU
+oracle: <= 1e-13 relative (max-norm) in FP64; FP32 and BF16 storage within
+their rounding (stated per test)."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import phi_field, spinor
+
+pytestmark = pytest.mark.gpu
+
+fm = pytest.importorskip("chiralheisenberg_amd.fermion")
+
+_lats = {}
+
+
+def lattices(L, Ls, Lt):
+    key = (L, Ls, Lt)
+    if key not in _lats:
+        _lats[key] = (fm.Lattice(L, Ls, Lt), orc.Lattice(L, Ls, Lt))
+    return _lats[key]
+
+
+SIZES = [(2, 2, 2), (4, 4, 4), (8, 8, 8), (6, 5, 4), (12, 8, 12), (12, 24, 12)]
+
+
+@pytest.mark.parametrize("L,Ls,Lt", SIZES)
+@pytest.mark.parametrize("imass", [1, 3, 2])
+def test_dslash_fp64_parity(L, Ls, Lt, imass):
+    lat, olat = lattices(L, Ls, Lt)
+    rs = np.random.RandomState(1000 * L + 10 * Ls + imass)
+    R = spinor(rs, lat.shape)
+    chi = phi_field(rs, lat.kvol)
+    for dag in (False, True):
+        for hpin in (0.0, 0.5):
+            g = (fm.dslashd if dag else fm.dslash)(lat, R, chi, 0.01, hpin, imass)
+            c = orc.dslash(olat, R, chi, hpin, imass, dag)
+            err = np.abs(g - c).max() / np.abs(c).max()
+            assert err <= 1e-13, (L, Ls, Lt, imass, dag, hpin, err)
+
+
+def test_dslash_adjoint_gpu():
+    lat, _ = lattices(8, 8, 8)
+    rs = np.random.RandomState(5)
+    X, Y = spinor(rs, lat.shape), spinor(rs, lat.shape)
+    chi = phi_field(rs, lat.kvol)
+    for imass in (1, 3):
+        lhs = np.vdot(Y, fm.dslash(lat, X, chi, 0.0, 0.7, imass))
+        rhs = np.vdot(fm.dslashd(lat, Y, chi, 0.0, 0.7, imass), X)
+        assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+
+
+def test_am_has_no_effect():
+    lat, _ = lattices(4, 4, 4)
+    rs = np.random.RandomState(6)
+    R, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    a = fm.dslash(lat, R, chi, 0.0, 0.0, 1)
+    b = fm.dslash(lat, R, chi, 0.37, 0.0, 1)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-6), ("bf16", 2e-2)])
+@pytest.mark.parametrize("dagger", [False, True])
+def test_dslash_low_precision(prec, tol, dagger):
+    """FP32 storage/compute: within a few FP32 ulps of the operator norm;
+    BF16 storage (8-bit mantissa) with FP32 accumulation: within ~2^-7
+    relative in the 2-norm."""
+    lat, olat = lattices(12, 24, 12)
+    rs = np.random.RandomState(7)
+    R, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    chi_d = fm.DeviceArray.from_numpy(lat, chi)
+    rin = fm.spinor_dev(lat, R, prec)
+    out = fm.spinor_dev(lat, None, prec)
+    fm.dslash_dev(lat, out, rin, chi_d, 0.0, 1, dagger)
+    g = fm.to_host(lat, out)
+    c = orc.dslash(olat, R, chi, 0.0, 1, dagger)
+    err = np.linalg.norm(g - c) / np.linalg.norm(c)
+    assert err <= tol, (prec, err)
+
+
+def test_dslash_rejects_alias():
+    lat, _ = lattices(4, 4, 4)
+    d = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+    chi = fm.DeviceArray.from_numpy(lat, np.zeros((3, lat.kvol)))
+    with pytest.raises(fm._lib.ChsError):
+        fm.dslash_dev(lat, d, d, chi)

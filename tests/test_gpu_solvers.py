@@ -1,0 +1,109 @@
+"""GPU parity of the solvers, force and Hamiltonian against the CPU oracle.
+
+Tolerances: solutions of the multi-shift solve and CG agree to the solver
+tolerance; iteration counts agree to within a few iterations (the stopping
+tests are thresholds on FP64 sums whose summation order differs between the
+GPU tree reductions and the oracle's chunked sums); forces and H to 1e-10
+relative when both sides run the force solves at a tight tolerance."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import phi_field, spinor
+
+pytestmark = pytest.mark.gpu
+
+fm = pytest.importorskip("chiralheisenberg_amd.fermion")
+from chiralheisenberg_amd import remez  # noqa: E402
+
+R4, R2, R2G = remez.load("remez4"), remez.load("remez2"), remez.load("remez2g")
+_lats = {}
+
+
+def lattices(L, Ls, Lt=None):
+    key = (L, Ls, Lt)
+    if key not in _lats:
+        _lats[key] = (fm.Lattice(L, Ls, Lt), orc.Lattice(L, Ls, Lt))
+    return _lats[key]
+
+
+@pytest.mark.parametrize("L,Ls", [(4, 4), (8, 8)])
+@pytest.mark.parametrize("imass", [1, 3])
+def test_qmrherm_iflag0(L, Ls, imass):
+    lat, olat = lattices(L, Ls)
+    rs = np.random.RandomState(20 + L + imass)
+    phi, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    a, ad = R4[0], R4[1]
+    xg, ig = fm.qmrherm(lat, phi, 1e-9, a, ad, chi, 0.0, 0.3, imass, 0)
+    xc, ic = orc.qmrherm(olat, phi, 1e-9, a, ad, chi, 0.3, imass, 0)
+    assert abs(ig - ic) <= 5, (ig, ic)
+    err = np.abs(xg - xc).max() / np.abs(xc).max()
+    assert err < 1e-9, err
+
+
+def test_qmrherm_iflag1_force_accumulates():
+    lat, olat = lattices(8, 8)
+    rs = np.random.RandomState(31)
+    phi, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    a, ad = R2G[0], R2G[1]
+    F0 = rs.randn(3, lat.kvol)
+    Fg, Fc = F0.copy(), F0.copy()
+    xg, ig = fm.qmrherm(lat, phi, 1e-12, a, ad, chi, 0.0, 0.0, 1, 1, Fg)
+    xc, ic = orc.qmrherm(olat, phi, 1e-12, a, ad, chi, 0.0, 1, 1, Fc)
+    assert np.allclose(xg, a[0] * phi, rtol=0, atol=1e-15)
+    err = np.abs(Fg - Fc).max() / np.abs(Fc - F0).max()
+    assert err < 1e-10, err
+
+
+def test_congrad_reference_rule():
+    lat, olat = lattices(8, 8)
+    rs = np.random.RandomState(41)
+    phi, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    x0 = 0.1 * spinor(rs, lat.shape)
+    xg, ig = fm.congrad(lat, phi, 1e-9, chi, x0)
+    xc, ic = orc.congrad(olat, phi, 1e-9, chi, x0)
+    assert abs(ig - ic) <= 3, (ig, ic)
+    err = np.abs(xg - xc).max() / np.abs(xc).max()
+    assert err < 1e-6, err
+
+
+@pytest.mark.parametrize("imass", [1, 3])
+def test_force_parity(imass):
+    """Force solves at res 1e-12 so that the solver tolerance is far below
+    the 1e-10 comparison."""
+    lat, olat = lattices(6, 6)
+    rs = np.random.RandomState(51 + imass)
+    phi, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    Fg, xg, itg = fm.force(lat, phi, 1e-12, chi, 0.475, R2G, 0.0, 0.2, imass)
+    Fc, xc, itc = orc.force(olat, phi, 1e-12, chi, 0.475, R2G, 0.2, imass)
+    err = np.abs(Fg - Fc).max() / np.abs(Fc).max()
+    assert err < 1e-10, (err, itg, itc)
+
+
+def test_hamilton_parity():
+    lat, olat = lattices(8, 8)
+    rs = np.random.RandomState(61)
+    phi, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    pp = rs.randn(3, lat.kvol)
+    hg, _, itg = fm.hamilton(lat, phi, 1e-11, chi, pp, 0.475, R2)
+    hc, _, itc = orc.hamilton(olat, phi, 1e-11, chi, pp, 0.475, R2)
+    for a, b in zip(hg, hc):
+        assert abs(a - b) <= 1e-10 * abs(b), (hg, hc)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+def test_solve_mdagm_true_residual(prec):
+    """Every form reaches an FP64 true relative residual <= 1e-10, checked
+    here independently with the oracle's operator."""
+    lat, olat = lattices(8, 8)
+    rs = np.random.RandomState(71)
+    b, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    bd = fm.spinor_dev(lat, b)
+    xd = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+    chid = fm.DeviceArray.from_numpy(lat, chi)
+    st = fm.solve_mdagm(lat, bd, xd, chid, 1e-10, prec)
+    x = fm.to_host(lat, xd)
+    r = b - orc.dslash(olat, orc.dslash(olat, x, chi, 0.0, 1, False), chi, 0.0, 1, True)
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    assert rel <= 1.05e-10, (prec, rel, st)
+    assert abs(st["true_resid"] - rel) <= 1e-3 * rel + 1e-16
