@@ -148,7 +148,7 @@ def main():
         "dslash_us": {"fp64": ms * 1e3, "fp32": ms32 * 1e3, "bf16": ms16 * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_dslash<F64,F64,false,EPI_STORE>", "bytes_per_launch": 2 * V + C},
+                     "kernel": "k_dsl<F64,F64,false,EPI_STORE>", "bytes_per_launch": 2 * V + C},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.L, args.Ls, args.beta, st64["iterations"])

@@ -60,6 +60,7 @@ def _declare(L):
         "chs_upload": ([v, v, v, sz], i),
         "chs_download": ([v, v, v, sz], i),
         "chs_convert_dev": ([v, v, i, v, i, l], i),
+        "chs_layout_dev": ([v, v, v, i, i], i),
         "chs_dslash_dev": ([v, v, v, v, d, i, i, i], i),
         "chs_qmrherm_dev": ([v, v, d, ip, d, i, v, v, i, i, v, v, v], i),
         "chs_congrad_dev": ([v, v, d, ip, d, i, v, v], i),

@@ -9,7 +9,7 @@
 // fixed sequences of kernels whose scalars stay on the device; the host
 // enqueues several iterations ahead and polls a halt flag, so no host round
 // trip sits inside an iteration.
-#include "chs_device.h"
+#include "chs_stencil.h"
 #include "../../include/chs.h"
 
 #include <chrono>
@@ -31,7 +31,7 @@ struct chs_ctx {
     std::string err;
     // device work space, grown on demand and kept
     struct Buf { void *p = nullptr; size_t n = 0; };
-    Buf b[16];
+    Buf b[20];
     Buf shifts_p, shifts_x, contrib;
     double *partial = nullptr, *partial2 = nullptr;
     int npart = 0;
@@ -47,7 +47,7 @@ static thread_local std::string g_create_err;
 
 namespace {
 
-enum BufId { B_PHI = 0, B_X, B_R, B_Q0, B_Q1, B_V, B_W, B_CHI, B_F, B_PP, B_P, B_T1, B_T2, B_XS, B_RS, B_PS };
+enum BufId { B_PHI = 0, B_X, B_R, B_Q0, B_Q1, B_V, B_W, B_CHI, B_F, B_PP, B_P, B_T1, B_T2, B_XS, B_RS, B_PS, B_STAGE };
 
 int fail(chs_ctx *c, int code, const char *fmt, ...) {
     char buf[512];
@@ -102,13 +102,36 @@ int ensure_partials(chs_ctx *c) {
     return CHS_OK;
 }
 
+// Host spinors are in the reference order (s fastest inside a plane); device
+// spinors in the native order (s outermost, chs_stencil.h).  Both directions
+// go through a staging buffer and one relayout kernel.
+int h2d_spinor(chs_ctx *c, void *dst, const void *host) {
+    int rc = ensure(c, c->b[B_STAGE], 16 * (size_t)c->kferm);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->b[B_STAGE].p, host, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL((k_relayout<F64>), dim3((unsigned)((c->kferm + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, c->lat, (double2 *)dst, (const double2 *)c->b[B_STAGE].p, 0);
+    HIPCHK(c, hipGetLastError());
+    return CHS_OK;
+}
+
+int d2h_spinor(chs_ctx *c, void *host, const void *src) {
+    int rc = ensure(c, c->b[B_STAGE], 16 * (size_t)c->kferm);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_relayout<F64>), dim3((unsigned)((c->kferm + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, c->lat, (double2 *)c->b[B_STAGE].p, (const double2 *)src, 1);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(host, c->b[B_STAGE].p, 16 * c->kferm, hipMemcpyDeviceToHost, c->stream));
+    return CHS_OK;
+}
+
 size_t prec_bytes(int prec) { return prec == CHS_FP64 ? 16 : prec == CHS_FP32 ? 8 : 4; }
 
 // ---- launchers --------------------------------------------------------------
 template <class S, class SA, bool DAG, int EPI>
 void dslash_launch(chs_ctx *c, void *out, const void *in, const double *chi, double hpin, int imass,
                    EpiArgs ea) {
-    hipLaunchKernelGGL((k_dslash<S, SA, DAG, EPI>), dim3(nblocks(c->n5)), dim3(kBlock), 0, c->stream, c->lat,
+    hipLaunchKernelGGL((k_dsl<S, SA, DAG, EPI>), dim3(stencil_grid(c->n5)), dim3(kBlock), 0, c->stream, c->lat,
                        (typename S::st *)out, (const typename S::st *)in, chi, hpin, imass, ea);
 }
 
@@ -205,6 +228,7 @@ chs_ctx *chs_ctx_create(int L, int Lt, int Ls, int ibound, double am3, int devic
     c->lat.n5 = c->n5;
     c->lat.diag = 4.0 - am3;
     c->lat.apbc = (ibound == -1);
+    c->lat.ps = c->n5;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->lz, sizeof(LzState)) != hipSuccess || hipMalloc(&c->cg, sizeof(CgState)) != hipSuccess ||
         hipMalloc(&c->dscal, 64 * sizeof(double)) != hipSuccess ||
@@ -298,6 +322,22 @@ int chs_convert_dev(chs_ctx *c, void *dst, int dp, const void *src, int sp, long
     return CHS_OK;
 }
 
+int chs_layout_dev(chs_ctx *c, void *dst, const void *src, int prec, int to_native) {
+    if (prec < 0 || prec > 2) return fail(c, CHS_EINVAL, "precision %d not in {0,1,2}", prec);
+    if (dst == src) return fail(c, CHS_EINVAL, "layout conversion cannot work in place");
+    const dim3 g((unsigned)((c->kferm + kBlock - 1) / kBlock)), b(kBlock);
+    const int dir = to_native ? 0 : 1;
+    if (prec == CHS_FP64)
+        hipLaunchKernelGGL((k_relayout<F64>), g, b, 0, c->stream, c->lat, (double2 *)dst, (const double2 *)src, dir);
+    else if (prec == CHS_FP32)
+        hipLaunchKernelGGL((k_relayout<F32>), g, b, 0, c->stream, c->lat, (float2 *)dst, (const float2 *)src, dir);
+    else
+        hipLaunchKernelGGL((k_relayout<B16>), g, b, 0, c->stream, c->lat, (uint32_t *)dst, (const uint32_t *)src,
+                           dir);
+    HIPCHK(c, hipGetLastError());
+    return CHS_OK;
+}
+
 // ---- operators ------------------------------------------------------------------
 int chs_dslash_dev(chs_ctx *c, void *out, const void *in, const double *chi, double hpin, int imass, int dagger,
                    int prec) {
@@ -315,11 +355,11 @@ static int dslash_host(chs_ctx *c, double *phi, const double *r, const double *c
     if (rc) return rc;
     rc = ensure(c, c->b[B_CHI], sizeof(double) * 3 * c->kvol);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->b[B_R].p, r, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    if ((rc = h2d_spinor(c, c->b[B_R].p, r))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->b[B_CHI].p, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     rc = chs_dslash_dev(c, c->b[B_V].p, c->b[B_R].p, buf<double>(c, B_CHI), hpin, imass, dagger, CHS_FP64);
     if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(phi, c->b[B_V].p, 16 * c->kferm, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = d2h_spinor(c, phi, c->b[B_V].p))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHS_OK;
 }
@@ -366,7 +406,7 @@ int chs_qmrherm_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
                        std::sqrt((double)n * res * res));
 
     const int niterc = 5000;
-    const int np1 = (int)nblocks(c->n5), np2 = (int)nblocks(n);
+    const int np1 = (int)stencil_grid(c->n5), np2 = (int)nblocks(n);
     const int chunk = chunk_for(c, (12.0 + 4.0 * ndiag) * 16.0 * n);
     const int *skip = &c->lz->done;
     int done = 0, iter = 1;
@@ -427,7 +467,7 @@ int chs_qmrherm(chs_ctx *c, const double *phi, double res, int *itercg, double a
     if (rc) return rc;
     if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
     if ((rc = ensure(c, c->b[B_F], 24 * c->kvol))) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->b[B_PHI].p, phi, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    if ((rc = h2d_spinor(c, c->b[B_PHI].p, phi))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->b[B_CHI].p, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     if (iflag == 1) {
         if (!dsdchi) return fail(c, CHS_EINVAL, "iflag=1 needs dsdchi");
@@ -436,7 +476,7 @@ int chs_qmrherm(chs_ctx *c, const double *phi, double res, int *itercg, double a
     rc = chs_qmrherm_dev(c, buf<double>(c, B_PHI), res, itercg, hpin, imass, anum, aden, ndiag, iflag,
                          buf<double>(c, B_CHI), buf<double>(c, B_X), buf<double>(c, B_F));
     if (rc) return rc;
-    if (x) HIPCHK(c, hipMemcpyAsync(x, c->b[B_X].p, 16 * c->kferm, hipMemcpyDeviceToHost, c->stream));
+    if (x && (rc = d2h_spinor(c, x, c->b[B_X].p))) return rc;
     if (iflag == 1) HIPCHK(c, hipMemcpyAsync(dsdchi, c->b[B_F].p, 24 * c->kvol, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHS_OK;
@@ -458,7 +498,7 @@ int chs_congrad_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
     HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(p, x, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(r, phi, 16 * n, hipMemcpyDeviceToDevice, c->stream));
-    const int np = (int)nblocks(c->n5);
+    const int np = (int)stencil_grid(c->n5);
     const int *skip = &c->cg->halt;
     const int chunk = chunk_for(c, 11.0 * 16.0 * n);
     int halt = 0, nx = 1;
@@ -501,13 +541,13 @@ int chs_congrad(chs_ctx *c, const double *phi, double res, int *itercg, double a
     int rc = ensure_spinors(c, {B_PHI, B_X});
     if (rc) return rc;
     if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->b[B_PHI].p, phi, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->b[B_X].p, x, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    if ((rc = h2d_spinor(c, c->b[B_PHI].p, phi))) return rc;
+    if ((rc = h2d_spinor(c, c->b[B_X].p, x))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->b[B_CHI].p, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     int rc2 = chs_congrad_dev(c, buf<double>(c, B_PHI), res, itercg, hpin, imass, buf<double>(c, B_CHI),
                               buf<double>(c, B_X));
     if (rc2 != CHS_OK && rc2 != CHS_ENOCONV) return rc2;
-    HIPCHK(c, hipMemcpyAsync(x, c->b[B_X].p, 16 * c->kferm, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = d2h_spinor(c, x, c->b[B_X].p))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return rc2;
 }
@@ -523,7 +563,7 @@ int chs_force(chs_ctx *c, const double *phi, double res1, double am, double hpin
     if ((rc = ensure(c, c->b[B_F], 24 * c->kvol))) return rc;
     double *dphi = buf<double>(c, B_PHI), *dx = buf<double>(c, B_X), *dx2 = buf<double>(c, B_T2);
     double *dchi = buf<double>(c, B_CHI), *dF = buf<double>(c, B_F);
-    HIPCHK(c, hipMemcpyAsync(dphi, phi, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    if ((rc = h2d_spinor(c, dphi, phi))) return rc;
     HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_force_init, dim3(nblocks(3 * c->kvol)), dim3(kBlock), 0, c->stream, 3 * c->kvol, beta,
                        (const double *)dchi, dF);
@@ -536,7 +576,7 @@ int chs_force(chs_ctx *c, const double *phi, double res1, double am, double hpin
     rc = chs_qmrherm_dev(c, dx2, res1, &i2, hpin, imass, anum, aden, ndiag, 1, dchi, dx, dF);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(dsdchi, dF, 24 * c->kvol, hipMemcpyDeviceToHost, c->stream));
-    if (x) HIPCHK(c, hipMemcpyAsync(x, dx, 16 * c->kferm, hipMemcpyDeviceToHost, c->stream));
+    if (x && (rc = d2h_spinor(c, x, dx))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (its) {
         its[0] = i1;
@@ -549,8 +589,8 @@ int chs_derivs(chs_ctx *c, const double *r, const double *x2, double anum, int i
     int rc = ensure_spinors(c, {B_T1, B_T2});
     if (rc) return rc;
     if ((rc = ensure(c, c->b[B_F], 24 * c->kvol))) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->b[B_T1].p, r, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->b[B_T2].p, x2, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    if ((rc = h2d_spinor(c, c->b[B_T1].p, r))) return rc;
+    if ((rc = h2d_spinor(c, c->b[B_T2].p, x2))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->b[B_F].p, dsdchi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_derivs, dim3(nblocks(c->kvol)), dim3(kBlock), 0, c->stream, c->lat,
                        (const double2 *)c->b[B_T1].p, (const double2 *)c->b[B_T2].p, anum, imass,
@@ -572,7 +612,7 @@ int chs_hamilton(chs_ctx *c, const double *phi, double *h, double *hg, double *h
     if ((rc = ensure(c, c->b[B_PP], 24 * c->kvol))) return rc;
     double *dphi = buf<double>(c, B_PHI), *dx = buf<double>(c, B_X), *dchi = buf<double>(c, B_CHI);
     double *dpp = buf<double>(c, B_PP);
-    HIPCHK(c, hipMemcpyAsync(dphi, phi, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
+    if ((rc = h2d_spinor(c, dphi, phi))) return rc;
     HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(dpp, pp, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     // sums of p^2 and chi^2 (rhmc/cheisdwf.f:546-559), FP64 on the device
@@ -584,7 +624,7 @@ int chs_hamilton(chs_ctx *c, const double *phi, double *h, double *hg, double *h
     reduce_launch(c, c->kferm, dphi, dx, 1, c->dscal + 10);   // Re Phi^dagger X (rhmc/cheisdwf.f:578-580)
     double sums[3];
     HIPCHK(c, hipMemcpyAsync(sums, c->dscal + 8, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (x) HIPCHK(c, hipMemcpyAsync(x, dx, 16 * c->kferm, hipMemcpyDeviceToHost, c->stream));
+    if (x && (rc = d2h_spinor(c, x, dx))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const double vhp = 0.5 * sums[0], vhg = beta * sums[1], vhf = sums[2];
     *hp = vhp;
@@ -622,7 +662,7 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
     const double2 *b = (const double2 *)bvec;
     double2 *x = (double2 *)xvec;
     const unsigned gn = nblocks(n);
-    const int np = (int)nblocks(c->n5);
+    const int np = (int)stencil_grid(c->n5);
 
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     reduce_launch(c, n, b, nullptr, 0, c->dscal);

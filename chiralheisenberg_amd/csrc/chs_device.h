@@ -4,15 +4,10 @@
 // (rhmc/cheisdwf.f:1098-1148), the wall-only force (rhmc/cheisdwf.f:733-849)
 // and deterministic FP64 reductions.
 //
-// Data layout in HBM: the reference layout itself.  A spinor is 8 planes
-// (Dirac d = 0..3, flavour f = 0..1, plane index d + 4 f), each plane holding
-// Ls*kvol complex numbers with s fastest, then x, y, t.  That is a
-// structure-of-arrays over the 8 spin-flavour components with the fifth
-// dimension contiguous: a wavefront of 64 consecutive lanes (s, x) streams
-// 64 consecutive complex numbers of every plane, and the x, y, t and s
-// neighbours of a lane are fixed offsets (Ls, Ls*L, Ls*L*L, 1) into the same
-// planes, so neighbour loads hit the lines the neighbouring lanes and
-// workgroups just brought into L1/L2.
+// Spinors are 8 spin-flavour planes (d + 4 f) of Ls*kvol complex numbers;
+// the order inside a plane (the "native" layout, s outermost) and the stencil
+// itself are in chs_stencil.h.  Everything here is elementwise over planes or
+// acts on the scalar state of the solvers.
 //
 // Storage precisions: FP64 (double2), FP32 (float2), BF16 (re, im packed in
 // one 32-bit word).  FP64 storage computes in FP64; FP32 and BF16 storage
@@ -30,6 +25,7 @@ struct Lat {
     int L, Lt, Ls;
     int kvol;       // L*L*Lt
     long n5;        // Ls*kvol: complex numbers per spin-flavour plane
+    long ps;        // plane stride in elements (>= n5)
     double diag;    // 4 - M5
     int apbc;       // antiperiodic in t (ibound == -1)
 };
@@ -105,45 +101,6 @@ __device__ __forceinline__ double block_sum(double v) {
     return t;
 }
 
-// ---- the operator at one 5D site ------------------------------------------
-// Adds the hop  -1/2 bc (1 - e g_mu) psi(nb)  to o, e = +1 for the M forward
-// hop; the backward hop and M^dagger flip e.  The projector keeps two
-// independent combinations a, b (rhmc/cheisdwf.f:2671-2828 for M,
-// :3108-3265 for M^dagger).
-template <class S, int MU>
-__device__ __forceinline__ void hop(cx<typename S::rt> (&o)[4][2], const typename S::st *__restrict__ in,
-                                    long gn, long P, typename S::rt e, typename S::rt coef) {
-    typedef typename S::rt T;
-#pragma unroll
-    for (int f = 0; f < 2; f++) {
-        const cx<T> p0 = S::ld(in, gn + P * (0 + 4 * f));
-        const cx<T> p1 = S::ld(in, gn + P * (1 + 4 * f));
-        const cx<T> p2 = S::ld(in, gn + P * (2 + 4 * f));
-        const cx<T> p3 = S::ld(in, gn + P * (3 + 4 * f));
-        cx<T> a, b, h2, h3;
-        if (MU == 0) {          // g1: a = p0 + e i p3, b = p1 + e i p2 -> (a, b, -e i b, -e i a)
-            a = cadd(p0, cmuli(e, p3));
-            b = cadd(p1, cmuli(e, p2));
-            h2 = cmuli(-e, b);
-            h3 = cmuli(-e, a);
-        } else if (MU == 1) {   // g2: a = p0 + e p3, b = p1 - e p2 -> (a, b, -e b, e a)
-            a = cadd(p0, cscale(e, p3));
-            b = csub(p1, cscale(e, p2));
-            h2 = cscale(-e, b);
-            h3 = cscale(e, a);
-        } else {                // g3: a = p0 + e i p2, b = p1 - e i p3 -> (a, b, -e i a, e i b)
-            a = cadd(p0, cmuli(e, p2));
-            b = csub(p1, cmuli(e, p3));
-            h2 = cmuli(-e, a);
-            h3 = cmuli(e, b);
-        }
-        o[0][f] = cadd(o[0][f], cscale(coef, a));
-        o[1][f] = cadd(o[1][f], cscale(coef, b));
-        o[2][f] = cadd(o[2][f], cscale(coef, h2));
-        o[3][f] = cadd(o[3][f], cscale(coef, h3));
-    }
-}
-
 // W psi for the wall block W = c * (phi.sigma [+ hpin sigma3 at site 0]):
 // imass=1: c = 1 on both walls (rhmc/cheisdwf.f:2912-2995);
 // imass=3: c = -i on the s=Ls wall, +i on the s=1 wall, no pinning
@@ -164,103 +121,7 @@ __device__ __forceinline__ void wall_apply(cx<T> &o0, cx<T> &o1, cx<T> v0, cx<T>
     o1 = cadd(o1, w1);
 }
 
-// o = (M in)(s, site) (DAG = false) or (M^dagger in)(s, site); cen receives
-// the centre values in[s, site] (used by fused epilogues).
-template <class S, bool DAG>
-__device__ __forceinline__ void site_apply(const Lat &lat, const typename S::st *__restrict__ in,
-                                           const double *__restrict__ chi, double hpin, int imass, int s,
-                                           int site, cx<typename S::rt> (&o)[4][2],
-                                           cx<typename S::rt> (&cen)[4][2]) {
-    typedef typename S::rt T;
-    const int L = lat.L, Ls = lat.Ls;
-    const long P = lat.n5;
-    const int ix = site % L;
-    const int r1 = site / L;
-    const int iy = r1 % L;
-    const int it = r1 / L;
-    const long g = (long)site * Ls + s;
-    const T dg = (T)lat.diag;
-#pragma unroll
-    for (int f = 0; f < 2; f++)
-#pragma unroll
-        for (int d = 0; d < 4; d++) {
-            cen[d][f] = S::ld(in, g + P * (d + 4 * f));
-            o[d][f] = cscale(dg, cen[d][f]);
-        }
-    const T e = DAG ? T(-1) : T(1);
-    const T mh = T(-0.5);
-    // x
-    {
-        const long gf = g + (long)Ls * ((ix + 1 == L) ? 1 - L : 1);
-        const long gb = g + (long)Ls * ((ix == 0) ? L - 1 : -1);
-        hop<S, 0>(o, in, gf, P, e, mh);
-        hop<S, 0>(o, in, gb, P, -e, mh);
-    }
-    // y
-    {
-        const long sy = (long)Ls * L;
-        const long gf = g + sy * ((iy + 1 == L) ? 1 - L : 1);
-        const long gb = g + sy * ((iy == 0) ? L - 1 : -1);
-        hop<S, 1>(o, in, gf, P, e, mh);
-        hop<S, 1>(o, in, gb, P, -e, mh);
-    }
-    // t, antiperiodic across it = Lt-1 <-> 0 when apbc
-    {
-        const long st = (long)Ls * L * L;
-        const int Lt = lat.Lt;
-        const long gf = g + st * ((it + 1 == Lt) ? 1 - Lt : 1);
-        const long gb = g + st * ((it == 0) ? Lt - 1 : -1);
-        const T cf = (lat.apbc && it + 1 == Lt) ? T(0.5) : mh;
-        const T cb = (lat.apbc && it == 0) ? T(0.5) : mh;
-        hop<S, 2>(o, in, gf, P, e, cf);
-        hop<S, 2>(o, in, gb, P, -e, cb);
-    }
-    // fifth dimension: M takes P- psi(s+1) and P+ psi(s-1); M^dagger the
-    // reverse (rhmc/cheisdwf.f:2879-2908, :3313-3342).  At s = Ls-1 (s = 0)
-    // the missing hop is replaced by the wall block acting on psi(0)
-    // (psi(Ls-1)).
-    const int dn = DAG ? 0 : 2;   // first component taking s+1
-    const int dp = DAG ? 2 : 0;   // first component taking s-1
-    const bool wall_ok = (imass == 1 || imass == 3);
-    T c1 = 0, c2 = 0, c3 = 0;
-    if ((s == 0 || s == Ls - 1) && wall_ok) {
-        c1 = (T)chi[site];
-        c2 = (T)chi[site + lat.kvol];
-        double c3d = chi[site + 2L * lat.kvol];
-        if (imass == 1 && site == 0) c3d += hpin;
-        c3 = (T)c3d;
-    }
-    {
-        const long gn = g + ((s + 1 == Ls) ? 1 - Ls : 1);
-#pragma unroll
-        for (int dd = 0; dd < 2; dd++) {
-            const cx<T> v0 = S::ld(in, gn + P * (dn + dd));
-            const cx<T> v1 = S::ld(in, gn + P * (dn + dd + 4));
-            if (s + 1 < Ls) {
-                o[dn + dd][0] = csub(o[dn + dd][0], v0);
-                o[dn + dd][1] = csub(o[dn + dd][1], v1);
-            } else if (wall_ok) {
-                wall_apply<T>(o[dn + dd][0], o[dn + dd][1], v0, v1, c1, c2, c3, imass, true);
-            }
-        }
-    }
-    {
-        const long gp = g + ((s == 0) ? Ls - 1 : -1);
-#pragma unroll
-        for (int dd = 0; dd < 2; dd++) {
-            const cx<T> v0 = S::ld(in, gp + P * (dp + dd));
-            const cx<T> v1 = S::ld(in, gp + P * (dp + dd + 4));
-            if (s > 0) {
-                o[dp + dd][0] = csub(o[dp + dd][0], v0);
-                o[dp + dd][1] = csub(o[dp + dd][1], v1);
-            } else if (wall_ok) {
-                wall_apply<T>(o[dp + dd][0], o[dp + dd][1], v0, v1, c1, c2, c3, imass, false);
-            }
-        }
-    }
-}
-
-// ---- fused dslash kernels ---------------------------------------------------
+// ---- epilogues of the fused stencil kernels (chs_stencil.h) ---------------
 enum Epi {
     EPI_STORE = 0,   // out = M in
     EPI_NORM = 1,    // out = M in, partial += |out|^2
@@ -277,78 +138,6 @@ struct EpiArgs {
     const int *skip;   // if not NULL and *skip != 0 the launch does nothing
     int flag;
 };
-
-template <class S, class SA, bool DAG, int EPI>
-__global__ __launch_bounds__(kBlock) void k_dslash(Lat lat, typename S::st *__restrict__ out,
-                                                   const typename S::st *__restrict__ in,
-                                                   const double *__restrict__ chi, double hpin, int imass,
-                                                   EpiArgs ea) {
-    typedef typename S::rt T;
-    if (ea.skip && *ea.skip) return;
-    const long lb = logical_block();
-    const long g = lb * kBlock + threadIdx.x;
-    double part = 0.0;
-    if (g < lat.n5) {
-        const int site = (int)(g / lat.Ls);
-        const int s = (int)(g - (long)site * lat.Ls);
-        cx<T> o[4][2], c[4][2];
-        site_apply<S, DAG>(lat, in, chi, hpin, imass, s, site, o, c);
-        const long P = lat.n5;
-        if (EPI == EPI_SCALEQ) {
-            const T ib = (T)(1.0 / ea.scal[0]);
-            typename S::st *q = (typename S::st *)ea.aux0;
-#pragma unroll
-            for (int f = 0; f < 2; f++)
-#pragma unroll
-                for (int d = 0; d < 4; d++) {
-                    S::sto(out, g + P * (d + 4 * f), cscale(ib, o[d][f]));
-                    S::sto(q, g + P * (d + 4 * f), cscale(ib, c[d][f]));
-                }
-        } else if (EPI == EPI_CG) {
-            const T alpha = (T)ea.scal[0];
-            const typename S::st *p = (const typename S::st *)ea.aux0;
-            typename SA::st *x = (typename SA::st *)ea.aux1;
-            typename SA::st *r = (typename SA::st *)ea.aux2;
-            T acc = 0;
-#pragma unroll
-            for (int f = 0; f < 2; f++)
-#pragma unroll
-                for (int d = 0; d < 4; d++) {
-                    const long i = g + P * (d + 4 * f);
-                    if (!(ea.flag & 1)) {
-                        cx<T> xv = SA::ld(x, i);
-                        const cx<T> pv = S::ld(p, i);
-                        xv = cadd(xv, cscale(alpha, pv));
-                        SA::sto(x, i, xv);
-                    }
-                    cx<T> rv = SA::ld(r, i);
-                    rv = csub(rv, cscale(alpha, o[d][f]));
-                    SA::sto(r, i, rv);
-                    acc += rv.r * rv.r + rv.i * rv.i;
-                }
-            part = (double)acc;
-        } else {
-            T acc = 0;
-            const typename S::st *a0 = (const typename S::st *)ea.aux0;
-#pragma unroll
-            for (int f = 0; f < 2; f++)
-#pragma unroll
-                for (int d = 0; d < 4; d++) {
-                    S::sto(out, g + P * (d + 4 * f), o[d][f]);
-                    if (EPI == EPI_NORM) acc += o[d][f].r * o[d][f].r + o[d][f].i * o[d][f].i;
-                    if (EPI == EPI_DOT) {
-                        const cx<T> av = S::ld(a0, g + P * (d + 4 * f));
-                        acc += av.r * o[d][f].r + av.i * o[d][f].i;
-                    }
-                }
-            part = (double)acc;
-        }
-    }
-    if (EPI == EPI_NORM || EPI == EPI_DOT || EPI == EPI_CG) {
-        const double t = block_sum(part);
-        if (threadIdx.x == 0) ea.partial[lb] = t;
-    }
-}
 
 // ---- deterministic FP64 reductions -----------------------------------------
 // One workgroup sums n partials in a fixed order.
@@ -583,42 +372,6 @@ __device__ __forceinline__ void derivs_site(double &F1, double &F2, double &F3, 
     }
 }
 
-// One thread per (site, shift): contribution of shift k to the force at the
-// site, from y = x_k and z = M x_k on the two wall slices only, into
-// contrib[(k*3 + a)*kvol + site].
-__global__ __launch_bounds__(kBlock) void k_force_shift(Lat lat, const double2 *__restrict__ xk, long n,
-                                                        int ndiag, const double *__restrict__ anum,
-                                                        const double *__restrict__ chi, double hpin, int imass,
-                                                        double *__restrict__ contrib) {
-    const long gid = logical_block() * kBlock + threadIdx.x;
-    if (gid >= (long)lat.kvol * ndiag) return;
-    const int k = (int)(gid / lat.kvol);
-    const int site = (int)(gid - (long)k * lat.kvol);
-    const double2 *y = xk + (long)k * n;
-    cx<double> zf[4][2], zl[4][2], cf[4][2], cl[4][2];
-    site_apply<F64, false>(lat, y, chi, hpin, imass, 0, site, zf, cf);
-    site_apply<F64, false>(lat, y, chi, hpin, imass, lat.Ls - 1, site, zl, cl);
-    double2 ya[4][2], za[4][2];
-#pragma unroll
-    for (int f = 0; f < 2; f++) {
-#pragma unroll
-        for (int d = 0; d < 2; d++) {   // y(s=Ls) with z(s=1)
-            ya[d][f] = make_double2(cl[d][f].r, cl[d][f].i);
-            za[d][f] = make_double2(zf[d][f].r, zf[d][f].i);
-        }
-#pragma unroll
-        for (int d = 2; d < 4; d++) {   // y(s=1) with z(s=Ls)
-            ya[d][f] = make_double2(cf[d][f].r, cf[d][f].i);
-            za[d][f] = make_double2(zl[d][f].r, zl[d][f].i);
-        }
-    }
-    double F1 = 0, F2 = 0, F3 = 0;
-    derivs_site(F1, F2, F3, ya, za, -2.0 * anum[k + 1], imass);
-    contrib[((long)k * 3 + 0) * lat.kvol + site] = F1;
-    contrib[((long)k * 3 + 1) * lat.kvol + site] = F2;
-    contrib[((long)k * 3 + 2) * lat.kvol + site] = F3;
-}
-
 // dSdchi[a][site] += sum_k contrib[k][a][site], in shift order
 __global__ __launch_bounds__(kBlock) void k_force_sum(int kvol, int ndiag, const double *__restrict__ contrib,
                                                       double *__restrict__ F) {
@@ -629,30 +382,6 @@ __global__ __launch_bounds__(kBlock) void k_force_sum(int kvol, int ndiag, const
     double acc = F[i];
     for (int k = 0; k < ndiag; k++) acc += contrib[((long)k * 3 + a) * kvol + site];
     F[i] = acc;
-}
-
-// derivs on given y, z (both full spinors), one thread per site
-__global__ __launch_bounds__(kBlock) void k_derivs(Lat lat, const double2 *__restrict__ y,
-                                                   const double2 *__restrict__ z, double anum, int imass,
-                                                   double *__restrict__ F) {
-    const long site = logical_block() * kBlock + threadIdx.x;
-    if (site >= lat.kvol) return;
-    const long P = lat.n5;
-    const long g0 = site * lat.Ls, gl = g0 + lat.Ls - 1;
-    double2 ya[4][2], za[4][2];
-#pragma unroll
-    for (int f = 0; f < 2; f++)
-#pragma unroll
-        for (int d = 0; d < 4; d++) {
-            const long gy = d < 2 ? gl : g0, gz = d < 2 ? g0 : gl;
-            ya[d][f] = y[gy + P * (d + 4 * f)];
-            za[d][f] = z[gz + P * (d + 4 * f)];
-        }
-    double F1 = F[site], F2 = F[site + lat.kvol], F3 = F[site + 2L * lat.kvol];
-    derivs_site(F1, F2, F3, ya, za, -2.0 * anum, imass);
-    F[site] = F1;
-    F[site + lat.kvol] = F2;
-    F[site + 2L * lat.kvol] = F3;
 }
 
 // F = 2 beta chi (rhmc/cheisdwf.f:494-498)

@@ -1,0 +1,343 @@
+// chs_stencil.h -- the production domain-wall stencil of libchs (M and
+// M^dagger of rhmc/cheisdwf.f:2634-3501) with fused solver epilogues.
+//
+// Device layout ("native"): a spinor is 8 spin-flavour planes (d + 4 f), each
+// plane Ls three-dimensional volumes: element s*kvol + site, site = x + L y +
+// L^2 t.  The reference layout keeps s fastest (site*Ls + s); the native one
+// keeps s outermost inside a plane, so that a run of consecutive workgroups
+// covers whole 3-D s-slices.  Workgroups go to the 8 XCDs in contiguous
+// ranges (logical_block_x), so one XCD owns a few complete s-slices of one
+// flavour: every x, y, t neighbour it reads is its own data, L2-resident
+// within a distance of L^2 elements, and only the two s-slices at the edge of
+// its range (read by the s-hop) and the walls are shared with another XCD.
+// With s fastest, an XCD owned a thin t-slab instead and re-fetched the slabs
+// on either side (measured: 2.0x the input bytes from beyond L2 in FP64 at
+// L=24, Ls=32).  The host boundary converts layouts once per call.
+//
+// Threads: one per (s, site, flavour) (NF = 1) with the flavour constant per
+// workgroup, so that every plane base is wave-uniform (scalar registers) and
+// a lane only carries a 32-bit element offset (contexts reject lattices with
+// 8*Ls*kvol >= 2^31).  Output stores are non-temporal: nothing re-reads the
+// output in the same launch, and keeping it out of L2 leaves L2 to the
+// neighbour reuse of the input.
+#pragma once
+#include "chs_device.h"
+
+namespace chs {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// XCD-contiguous logical block along x (see logical_block)
+__device__ __forceinline__ unsigned logical_block_x() {
+    const unsigned b = blockIdx.x, nb = gridDim.x;
+    if ((nb & 7u) == 0u) return (b & 7u) * (nb >> 3) + (b >> 3);
+    return b;
+}
+
+template <class S>
+__device__ __forceinline__ void store_nt(typename S::st *__restrict__ p, unsigned i, cx<typename S::rt> v) {
+    typename S::st w;
+    S::sto(&w, 0, v);
+    if constexpr (sizeof(w) == 16) {
+        u32x4 raw;
+        __builtin_memcpy(&raw, &w, 16);
+        __builtin_nontemporal_store(raw, reinterpret_cast<u32x4 *>(p + i));
+    } else if constexpr (sizeof(w) == 8) {
+        u32x2 raw;
+        __builtin_memcpy(&raw, &w, 8);
+        __builtin_nontemporal_store(raw, reinterpret_cast<u32x2 *>(p + i));
+    } else {
+        unsigned raw;
+        __builtin_memcpy(&raw, &w, 4);
+        __builtin_nontemporal_store(raw, reinterpret_cast<unsigned *>(p + i));
+    }
+}
+
+// o[ff] += coef * (1 - e g_mu) psi_ff(gn); pl[ff][d] are uniform plane bases.
+// MU = 0, 1, 2 (x, y, t) with e = +1; MU = -1, -2, -3 the same with e = -1.
+template <class S, int MU, int NF>
+__device__ __forceinline__ void hop2(cx<typename S::rt> (&o)[NF][4], const typename S::st *const (&pl)[NF][4],
+                                     unsigned gn, typename S::rt coef) {
+    typedef typename S::rt T;
+    constexpr int mu = MU < 0 ? -MU - 1 : MU;
+    constexpr T e = MU < 0 ? T(-1) : T(1);
+#pragma unroll
+    for (int ff = 0; ff < NF; ff++) {
+        const cx<T> p0 = S::ld(pl[ff][0], gn), p1 = S::ld(pl[ff][1], gn);
+        const cx<T> p2 = S::ld(pl[ff][2], gn), p3 = S::ld(pl[ff][3], gn);
+        cx<T> a, b, h2, h3;
+        if (mu == 0) {          // g1: (a, b, -e i b, -e i a), a = p0 + e i p3, b = p1 + e i p2
+            a = cadd(p0, cmuli(e, p3));
+            b = cadd(p1, cmuli(e, p2));
+            h2 = cmuli(-e, b);
+            h3 = cmuli(-e, a);
+        } else if (mu == 1) {   // g2: (a, b, -e b, e a), a = p0 + e p3, b = p1 - e p2
+            a = cadd(p0, cscale(e, p3));
+            b = csub(p1, cscale(e, p2));
+            h2 = cscale(-e, b);
+            h3 = cscale(e, a);
+        } else {                // g3: (a, b, -e i a, e i b), a = p0 + e i p2, b = p1 - e i p3
+            a = cadd(p0, cmuli(e, p2));
+            b = csub(p1, cmuli(e, p3));
+            h2 = cmuli(-e, a);
+            h3 = cmuli(e, b);
+        }
+        o[ff][0] = {o[ff][0].r + coef * a.r, o[ff][0].i + coef * a.i};
+        o[ff][1] = {o[ff][1].r + coef * b.r, o[ff][1].i + coef * b.i};
+        o[ff][2] = {o[ff][2].r + coef * h2.r, o[ff][2].i + coef * h2.i};
+        o[ff][3] = {o[ff][3].r + coef * h3.r, o[ff][3].i + coef * h3.i};
+    }
+}
+
+// Element offset of (s, site) inside a plane.
+template <int LAYOUT>
+__device__ __forceinline__ unsigned elem(unsigned s, unsigned site, unsigned Ls, unsigned kvol) {
+    return LAYOUT ? s * kvol + site : site * Ls + s;
+}
+
+// (M in)(s, site) or (M^dagger in)(s, site) for the flavours fb..fb+NF-1.
+// in: base of the whole spinor; c receives the centre values.
+template <class S, bool DAG, int NF, int LAYOUT>
+__device__ __forceinline__ void stencil(const Lat &lat, const typename S::st *__restrict__ in,
+                                        const double *__restrict__ chi, double hpin, int imass, unsigned s,
+                                        unsigned site, int fb, cx<typename S::rt> (&o)[NF][4],
+                                        cx<typename S::rt> (&c)[NF][4]) {
+    typedef typename S::rt T;
+    typedef typename S::st st;
+    const unsigned Ls = (unsigned)lat.Ls, L = (unsigned)lat.L, Lt = (unsigned)lat.Lt, kvol = (unsigned)lat.kvol;
+    const size_t P = (size_t)lat.ps;
+    const st *pl[NF][4];
+#pragma unroll
+    for (int ff = 0; ff < NF; ff++)
+#pragma unroll
+        for (int d = 0; d < 4; d++) pl[ff][d] = in + P * (size_t)(d + 4 * (fb + ff));
+    const unsigned ix = site % L;
+    const unsigned r1 = site / L;
+    const unsigned iy = r1 % L;
+    const unsigned it = r1 / L;
+    const unsigned g = elem<LAYOUT>(s, site, Ls, kvol);
+    const T dg = (T)lat.diag;
+#pragma unroll
+    for (int ff = 0; ff < NF; ff++)
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            c[ff][d] = S::ld(pl[ff][d], g);
+            o[ff][d] = cscale(dg, c[ff][d]);
+        }
+    const T mh = T(-0.5);
+    // M: forward (1 - g), backward (1 + g); M^dagger the reverse
+    constexpr int XF = DAG ? -1 : 0, XB = DAG ? 0 : -1;
+    constexpr int YF = DAG ? -2 : 1, YB = DAG ? 1 : -2;
+    constexpr int TF = DAG ? -3 : 2, TB = DAG ? 2 : -3;
+    const unsigned sx = LAYOUT ? 1u : Ls;
+    const unsigned sy = sx * L, stt = sy * L;
+    const unsigned ss = LAYOUT ? kvol : 1u;
+    hop2<S, XF, NF>(o, pl, (ix + 1 == L) ? g - (L - 1) * sx : g + sx, mh);
+    hop2<S, XB, NF>(o, pl, (ix == 0) ? g + (L - 1) * sx : g - sx, mh);
+    hop2<S, YF, NF>(o, pl, (iy + 1 == L) ? g - (L - 1) * sy : g + sy, mh);
+    hop2<S, YB, NF>(o, pl, (iy == 0) ? g + (L - 1) * sy : g - sy, mh);
+    // antiperiodic in t across it = Lt-1 <-> 0 (rhmc/cheisdwf.f:2831-2876)
+    hop2<S, TF, NF>(o, pl, (it + 1 == Lt) ? g - (Lt - 1) * stt : g + stt,
+                    (lat.apbc && it + 1 == Lt) ? T(0.5) : mh);
+    hop2<S, TB, NF>(o, pl, (it == 0) ? g + (Lt - 1) * stt : g - stt, (lat.apbc && it == 0) ? T(0.5) : mh);
+    // fifth dimension: M takes P- psi(s+1), P+ psi(s-1); M^dagger the reverse
+    // (rhmc/cheisdwf.f:2879-2908, :3313-3342).  At s = Ls-1 (s = 0) the
+    // missing hop is replaced by the wall block acting on psi(0) (psi(Ls-1)),
+    // both flavours (rhmc/cheisdwf.f:2912-3066, :3345-3498).
+    constexpr int dn = DAG ? 0 : 2;
+    constexpr int dp = DAG ? 2 : 0;
+    const bool wall_ok = (imass == 1 || imass == 3);
+    const bool last = (s + 1 == Ls), first = (s == 0);
+    if (!last) {
+#pragma unroll
+        for (int ff = 0; ff < NF; ff++) {
+            o[ff][dn] = csub(o[ff][dn], S::ld(pl[ff][dn], g + ss));
+            o[ff][dn + 1] = csub(o[ff][dn + 1], S::ld(pl[ff][dn + 1], g + ss));
+        }
+    }
+    if (!first) {
+#pragma unroll
+        for (int ff = 0; ff < NF; ff++) {
+            o[ff][dp] = csub(o[ff][dp], S::ld(pl[ff][dp], g - ss));
+            o[ff][dp + 1] = csub(o[ff][dp + 1], S::ld(pl[ff][dp + 1], g - ss));
+        }
+    }
+    if ((first || last) && wall_ok) {
+        const T c1 = (T)chi[site], c2 = (T)chi[site + kvol];
+        double c3d = chi[site + 2u * kvol];
+        if (imass == 1 && site == 0) c3d += hpin;
+        const T c3 = (T)c3d;
+        const st *q0 = in, *q1 = in + 4 * P;   // flavour 0 / 1 planes at d = 0
+        const unsigned src = last ? elem<LAYOUT>(0, site, Ls, kvol) : elem<LAYOUT>(Ls - 1, site, Ls, kvol);
+#pragma unroll
+        for (int dd = 0; dd < 2; dd++) {
+            const int d = last ? dn + dd : dp + dd;
+            const cx<T> v0 = S::ld(q0 + P * (size_t)d, src), v1 = S::ld(q1 + P * (size_t)d, src);
+            cx<T> w0 = {0, 0}, w1 = {0, 0};
+            wall_apply<T>(w0, w1, v0, v1, c1, c2, c3, imass, last);
+#pragma unroll
+            for (int ff = 0; ff < NF; ff++) {
+                const cx<T> w = (fb + ff) ? w1 : w0;
+                if (last) o[ff][dn + dd] = cadd(o[ff][dn + dd], w);
+                else o[ff][dp + dd] = cadd(o[ff][dp + dd], w);
+            }
+        }
+    }
+}
+
+// Workgroups per flavour for a stencil launch over n5 = Ls*kvol elements.
+__host__ __device__ inline unsigned stencil_blocks_per_flavour(long n5) {
+    return (unsigned)((n5 + kBlock - 1) / kBlock);
+}
+
+// out = M in / M^dagger in with the epilogues of chs::Epi.  1-D grid of
+// 2 * blocks_per_flavour workgroups rounded up to a multiple of 8; logical
+// block lb covers flavour lb / bpf, elements (lb % bpf) * kBlock + tid.
+// Partials: one per workgroup at partial[logical block].
+template <class S, class SA, bool DAG, int EPI, int LAYOUT = 1, int WPE = 1>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_dsl(
+    Lat lat, typename S::st *__restrict__ out, const typename S::st *__restrict__ in, const double *__restrict__ chi,
+    double hpin, int imass, EpiArgs ea) {
+    typedef typename S::rt T;
+    typedef typename SA::rt TA;
+    if (ea.skip && *ea.skip) return;
+    const unsigned lb = logical_block_x();
+    const unsigned bpf = stencil_blocks_per_flavour(lat.n5);
+    const int fb = (int)(lb / bpf);
+    const unsigned g = (lb - (unsigned)fb * bpf) * kBlock + threadIdx.x;   // position in (s, site) order
+    const size_t P = (size_t)lat.ps;
+    double part = 0.0;
+    if (fb < 2 && g < (unsigned)lat.n5) {
+        const unsigned Ls = (unsigned)lat.Ls, kvol = (unsigned)lat.kvol;
+        unsigned s, site;
+        if (LAYOUT) {
+            s = g / kvol;
+            site = g - s * kvol;
+        } else {
+            site = g / Ls;
+            s = g - site * Ls;
+        }
+        cx<T> o[1][4], c[1][4];
+        stencil<S, DAG, 1, LAYOUT>(lat, in, chi, hpin, imass, s, site, fb, o, c);
+        T acc = 0;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const size_t po = P * (size_t)(d + 4 * fb);
+            if (EPI == EPI_SCALEQ) {
+                const T ib = (T)(1.0 / ea.scal[0]);
+                store_nt<S>(out + po, g, cscale(ib, o[0][d]));
+                store_nt<S>((typename S::st *)ea.aux0 + po, g, cscale(ib, c[0][d]));
+            } else if (EPI == EPI_CG) {
+                const TA alpha = (TA)ea.scal[0];
+                typename SA::st *xr = (typename SA::st *)ea.aux1 + po;
+                typename SA::st *rr = (typename SA::st *)ea.aux2 + po;
+                if (!(ea.flag & 1)) {
+                    const cx<T> pv = S::ld((const typename S::st *)ea.aux0 + po, g);
+                    cx<TA> xv = SA::ld(xr, g);
+                    xv = {xv.r + alpha * (TA)pv.r, xv.i + alpha * (TA)pv.i};
+                    store_nt<SA>(xr, g, xv);
+                }
+                cx<TA> rv = SA::ld(rr, g);
+                rv = {rv.r - alpha * (TA)o[0][d].r, rv.i - alpha * (TA)o[0][d].i};
+                store_nt<SA>(rr, g, rv);
+                acc += rv.r * rv.r + rv.i * rv.i;
+            } else {
+                store_nt<S>(out + po, g, o[0][d]);
+                if (EPI == EPI_NORM) acc += o[0][d].r * o[0][d].r + o[0][d].i * o[0][d].i;
+                if (EPI == EPI_DOT) {
+                    const cx<T> av = S::ld((const typename S::st *)ea.aux0 + po, g);
+                    acc += av.r * o[0][d].r + av.i * o[0][d].i;
+                }
+            }
+        }
+        part = (double)acc;
+    }
+    if (EPI == EPI_NORM || EPI == EPI_DOT || EPI == EPI_CG) {
+        const double t = block_sum(part);
+        if (threadIdx.x == 0) ea.partial[lb] = t;
+    }
+}
+
+// Grid size (workgroups) of a k_dsl launch.
+inline unsigned stencil_grid(long n5) {
+    unsigned nb = 2 * stencil_blocks_per_flavour(n5);
+    return (nb + 7u) / 8u * 8u;
+}
+
+// ---- pseudofermion force on the two wall slices (rhmc/cheisdwf.f:733-849) --
+// One thread per (site, shift): y = x_k, z = M x_k on the slices s = 0 and
+// s = Ls-1 only (derivs uses no other slice), contribution of shift k at the
+// site into contrib[(k*3 + a)*kvol + site].  Native layout.
+__global__ __launch_bounds__(kBlock) void k_force_shift(Lat lat, const double2 *__restrict__ xk, long n,
+                                                        int ndiag, const double *__restrict__ anum,
+                                                        const double *__restrict__ chi, double hpin, int imass,
+                                                        double *__restrict__ contrib) {
+    const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (gid >= (long)lat.kvol * ndiag) return;
+    const int k = (int)(gid / lat.kvol);
+    const unsigned site = (unsigned)(gid - (long)k * lat.kvol);
+    const double2 *y = xk + (size_t)k * n;
+    cx<double> zf[2][4], cf[2][4], zl[2][4], cl[2][4];
+    stencil<F64, false, 2, 1>(lat, y, chi, hpin, imass, 0, site, 0, zf, cf);
+    stencil<F64, false, 2, 1>(lat, y, chi, hpin, imass, (unsigned)lat.Ls - 1, site, 0, zl, cl);
+    double2 ya[4][2], za[4][2];
+#pragma unroll
+    for (int f = 0; f < 2; f++)
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            // d = 0,1: y(s=Ls) with z(s=1); d = 2,3: y(s=1) with z(s=Ls)
+            const cx<double> yv = d < 2 ? cl[f][d] : cf[f][d];
+            const cx<double> zv = d < 2 ? zf[f][d] : zl[f][d];
+            ya[d][f] = make_double2(yv.r, yv.i);
+            za[d][f] = make_double2(zv.r, zv.i);
+        }
+    double F1 = 0, F2 = 0, F3 = 0;
+    derivs_site(F1, F2, F3, ya, za, -2.0 * anum[k + 1], imass);
+    contrib[((long)k * 3 + 0) * lat.kvol + site] = F1;
+    contrib[((long)k * 3 + 1) * lat.kvol + site] = F2;
+    contrib[((long)k * 3 + 2) * lat.kvol + site] = F3;
+}
+
+// derivs on given y and z (full native spinors), one thread per site:
+// F += wall bilinears (rhmc/cheisdwf.f:747-849)
+__global__ __launch_bounds__(kBlock) void k_derivs(Lat lat, const double2 *__restrict__ y,
+                                                   const double2 *__restrict__ z, double anum, int imass,
+                                                   double *__restrict__ F) {
+    const long site = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (site >= lat.kvol) return;
+    const size_t P = (size_t)lat.ps;
+    const size_t e0 = (size_t)site, el = (size_t)(lat.Ls - 1) * lat.kvol + site;
+    double2 ya[4][2], za[4][2];
+#pragma unroll
+    for (int f = 0; f < 2; f++)
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const size_t gy = d < 2 ? el : e0, gz = d < 2 ? e0 : el;
+            ya[d][f] = y[gy + P * (d + 4 * f)];
+            za[d][f] = z[gz + P * (d + 4 * f)];
+        }
+    double F1 = F[site], F2 = F[site + lat.kvol], F3 = F[site + 2L * lat.kvol];
+    derivs_site(F1, F2, F3, ya, za, -2.0 * anum, imass);
+    F[site] = F1;
+    F[site + lat.kvol] = F2;
+    F[site + 2L * lat.kvol] = F3;
+}
+
+// Layout conversion between the reference order (site*Ls + s) and the native
+// order (s*kvol + site) inside each of the 8 planes; dir = 0: ref -> native.
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_relayout(Lat lat, typename S::st *__restrict__ dst,
+                                                     const typename S::st *__restrict__ src, int dir) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;   // native order index over 8 planes
+    const size_t n5 = (size_t)lat.n5;
+    if (i >= 8 * n5) return;
+    const size_t plane = i / n5;
+    const unsigned e = (unsigned)(i - plane * n5);
+    const unsigned s = e / (unsigned)lat.kvol, site = e - s * (unsigned)lat.kvol;
+    const size_t r = plane * n5 + (size_t)site * lat.Ls + s;   // reference order index
+    if (dir == 0) dst[i] = src[r];
+    else dst[r] = src[i];
+}
+
+}  // namespace chs

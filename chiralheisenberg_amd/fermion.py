@@ -212,32 +212,42 @@ class DeviceArray:
 
 
 def spinor_dev(lat, a=None, precision="fp64"):
-    """Device spinor of the given storage precision, optionally filled from a
-    complex128 host array (converted on the device)."""
+    """Device spinor in the native layout and the given storage precision,
+    optionally filled from a complex128 host array in the reference layout
+    (reordered and converted on the device)."""
     prec = PRECISIONS[precision]
     d = DeviceArray(lat, lat.kferm * BYTES_PER_COMPLEX[prec])
     if a is not None:
         a = lat._spinor(a)
+        ref = DeviceArray.from_numpy(lat, a)
         if prec == CHS_FP64:
-            d.upload(a)
+            check(lat.h, lib().chs_layout_dev(lat.h, d.ptr, ref.ptr, CHS_FP64, 1))
         else:
-            tmp = DeviceArray.from_numpy(lat, a)
-            check(lat.h, lib().chs_convert_dev(lat.h, d.ptr, prec, tmp.ptr, CHS_FP64, lat.kferm))
+            nat = DeviceArray(lat, lat.kferm * 16)
+            check(lat.h, lib().chs_layout_dev(lat.h, nat.ptr, ref.ptr, CHS_FP64, 1))
+            check(lat.h, lib().chs_convert_dev(lat.h, d.ptr, prec, nat.ptr, CHS_FP64, lat.kferm))
             lat.synchronize()
-            tmp.free()
+            nat.free()
+        lat.synchronize()
+        ref.free()
     d.prec = prec
     return d
 
 
 def to_host(lat, d):
-    """complex128 host copy of a device spinor of any storage precision."""
+    """complex128 host copy, in the reference layout, of a native device
+    spinor of any storage precision."""
     prec = getattr(d, "prec", CHS_FP64)
-    if prec == CHS_FP64:
-        return d.download(np.complex128, lat.shape)
-    tmp = DeviceArray(lat, lat.kferm * 16)
-    check(lat.h, lib().chs_convert_dev(lat.h, tmp.ptr, CHS_FP64, d.ptr, prec, lat.kferm))
-    out = tmp.download(np.complex128, lat.shape)
-    tmp.free()
+    nat = d
+    if prec != CHS_FP64:
+        nat = DeviceArray(lat, lat.kferm * 16)
+        check(lat.h, lib().chs_convert_dev(lat.h, nat.ptr, CHS_FP64, d.ptr, prec, lat.kferm))
+    ref = DeviceArray(lat, lat.kferm * 16)
+    check(lat.h, lib().chs_layout_dev(lat.h, ref.ptr, nat.ptr, CHS_FP64, 0))
+    out = ref.download(np.complex128, lat.shape)
+    ref.free()
+    if nat is not d:
+        nat.free()
     return out
 
 

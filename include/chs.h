@@ -122,6 +122,14 @@ int chs_derivs(chs_ctx *ctx, const double *r, const double *x2, double anum,
                int imass, double *dsdchi);
 
 /* ---- device-resident interface ------------------------------------------ */
+/* Device spinors of the _dev entry points are in the library's NATIVE
+ * layout: the same 8 spin-flavour planes (d + 4 f), but inside a plane the
+ * element of (s, site) sits at s*kvol + site (s outermost) instead of the
+ * reference's site*Ls + s.  The stencil needs this to keep each XCD's
+ * neighbour reads inside its own L2.  chs_layout_dev converts between the two
+ * orders on the device (to_native = 1: reference -> native). */
+int chs_layout_dev(chs_ctx *ctx, void *dst, const void *src, int prec, int to_native);
+
 /* Device buffers allocated by the library (hipMalloc), freed with chs_free. */
 void *chs_alloc(chs_ctx *ctx, size_t bytes);
 int chs_free(chs_ctx *ctx, void *p);

@@ -1,0 +1,339 @@
+"""Pins the CPU oracle (oracle/chs_oracle.c) -- CPU only.
+
+1. Against the reference's own rhmc/cheisdwf.f: the committed golden vectors
+   (tests/golden/ref_l12_ls8.npz, written by tests/golden/make_golden.py from
+   the reference compiled unmodified with flang) and, when the reference
+   build oracle/_ref is present, directly.
+2. Against what the mathematics fixes, at sizes the reference cannot be
+   compiled for: the free-field operator in momentum space (closed form),
+   adjointness, dense solves and dense rational functions on tiny lattices,
+   a finite-difference derivative of the action for the force, and the
+   closed form of the 48-bit generator.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import refbind as RB
+from conftest import phi_field, spinor
+from chiralheisenberg_amd import remez
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = np.load(os.path.join(HERE, "golden", "ref_l12_ls8.npz"))
+R4, R2, R2G = remez.load("remez4"), remez.load("remez2"), remez.load("remez2g")
+
+
+def gold_inputs():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mkg", os.path.join(HERE, "golden", "make_golden.py"))
+    # the generator needs the reference build only in main(); inputs() is plain numpy
+    src = open(os.path.join(HERE, "golden", "make_golden.py")).read()
+    assert "def inputs" in src and spec is not None
+    rs = np.random.RandomState(int(GOLD["seed"]))
+    shape = (2, 4, RB.LT, RB.L, RB.L, RB.LS)
+    R = (rs.randn(*shape) + 1j * rs.randn(*shape)) / np.sqrt(2.0)
+    chi = rs.randn(3, RB.KVOL) / np.sqrt(0.475)
+    pp = rs.randn(3, RB.KVOL)
+    x0 = 0.05 * (rs.randn(*shape) + 1j * rs.randn(*shape))
+    idx = rs.choice(RB.KFERM, 512, replace=False)
+    assert np.array_equal(idx, GOLD["idx"])
+    return R, chi, pp, x0, idx
+
+
+LAT12 = orc.Lattice(12, 8)
+
+
+def close_sampled(a, name, rtol):
+    a = a.reshape(-1)
+    s, n = GOLD[name + "_s"], float(GOLD[name + "_n"])
+    return np.abs(a[GOLD["idx"]] - s).max() / np.abs(s).max() <= rtol and abs(np.linalg.norm(a) - n) <= rtol * n
+
+
+# ---- 1. reference vectors ---------------------------------------------------------------
+@pytest.mark.parametrize("imass", [1, 3])
+@pytest.mark.parametrize("dag", [0, 1])
+def test_dslash_vs_reference_golden(imass, dag):
+    R, chi, *_ = gold_inputs()
+    out = orc.dslash(LAT12, R, chi, 0.5, imass, bool(dag))
+    assert close_sampled(out, "dslash_m%d_d%d" % (imass, dag), 1e-14)
+
+
+def test_qmrherm_vs_reference_golden():
+    R, chi, *_ = gold_inputs()
+    x, it = orc.qmrherm(LAT12, R, 1e-9, R4[0], R4[1], chi, 0.0, 1, 0)
+    assert abs(it - int(GOLD["qmr4_it"])) <= 5
+    assert close_sampled(x, "qmr4", 1e-11)
+
+
+def test_qmrherm_force_vs_reference_golden():
+    R, chi, *_ = gold_inputs()
+    F = np.zeros((3, LAT12.kvol))
+    orc.qmrherm(LAT12, R, 1e-12, R2G[0], R2G[1], chi, 0.0, 1, 1, F)
+    G = GOLD["qmr2g_force"]
+    assert np.abs(F - G).max() <= 1e-10 * np.abs(G).max()
+
+
+def test_congrad_vs_reference_golden():
+    R, chi, pp, x0, _ = gold_inputs()
+    x, it = orc.congrad(LAT12, R, 1e-9, chi, x0)
+    assert abs(it - int(GOLD["congrad_it"])) <= 3
+    assert close_sampled(x, "congrad", 1e-7)
+
+
+def test_force_and_hamilton_vs_reference_golden():
+    R, chi, pp, *_ = gold_inputs()
+    F, _, _ = orc.force(LAT12, R, 1e-12, chi, 0.475, R2G)
+    G = GOLD["force"]
+    assert np.abs(F - G).max() <= 1e-10 * np.abs(G).max()
+    h, _, _ = orc.hamilton(LAT12, R, 1e-11, chi, pp, 0.475, R2)
+    for a, b in zip(h, GOLD["hamilton"]):
+        assert abs(a - b) <= 1e-11 * abs(b)
+
+
+def test_rng_vs_reference_golden():
+    seed = 693764523745.0
+    r = orc.Rng(seed)
+    assert np.array_equal(np.array([r.rranf() for _ in range(1000)]), GOLD["rranf"])
+    assert r.get() == float(GOLD["rranget"])
+    r = orc.Rng(seed)
+    assert np.array_equal(np.array([r.rano(0) for _ in range(201)]), GOLD["rano"])
+    r = orc.Rng(seed)
+    r.rano(0)
+    assert np.array_equal(r.gauss0(RB.KVOL), GOLD["gauss0"])
+    assert np.array_equal(r.gaussp(RB.KVOL), GOLD["gaussp"])
+
+
+@pytest.mark.skipif(not RB.available("fp64"), reason="reference build oracle/_ref absent")
+def test_dslash_vs_reference_build_direct():
+    ref = RB.Ref("fp64")
+    ref.set_params(0.45, 1.0, -1)
+    rs = np.random.RandomState(3)
+    R, chi = spinor(rs, LAT12.shape), phi_field(rs, LAT12.kvol)
+    for imass in (1, 3, 2):
+        for dag in (False, True):
+            a = orc.dslash(LAT12, R, chi, 0.7, imass, dag)
+            b = ref.dslash(R, chi, 0.7, imass, dag)
+            assert np.abs(a - b).max() <= 1e-14 * np.abs(b).max()
+
+
+@pytest.mark.skipif(not RB.available("shipped"), reason="reference build oracle/_ref absent")
+def test_shipped_semantics_is_fp32_chi():
+    """The shipped build rounds chi to FP32 inside dslash
+    (rhmc/cheisdwf.f:2917); the oracle fed FP32-rounded chi reproduces it."""
+    ref = RB.Ref("shipped")
+    ref.set_params(0.45, 1.0, -1)
+    rs = np.random.RandomState(4)
+    R, chi = spinor(rs, LAT12.shape), phi_field(rs, LAT12.kvol)
+    b = ref.dslash(R, chi, 0.5, 1, False)
+    a = orc.dslash(LAT12, R, chi.astype(np.float32).astype(np.float64), 0.5, 1, False)
+    assert np.abs(a - b).max() <= 1e-14 * np.abs(b).max()
+    a64 = orc.dslash(LAT12, R, chi, 0.5, 1, False)
+    assert np.abs(a64 - b).max() > 1e-9 * np.abs(b).max()
+
+
+# ---- 2. mathematics -----------------------------------------------------------------
+GAMMA = [
+    np.array([[0, 0, 0, -1j], [0, 0, -1j, 0], [0, 1j, 0, 0], [1j, 0, 0, 0]]),
+    np.array([[0, 0, 0, -1], [0, 0, 1, 0], [0, 1, 0, 0], [-1, 0, 0, 0]], dtype=complex),
+    np.array([[0, 0, -1j, 0], [0, 0, 0, 1j], [1j, 0, 0, 0], [0, -1j, 0, 0]]),
+]
+
+
+def test_gamma_algebra():
+    """The three gamma matrices of rhmc/cheisdwf.f:1523-1557 are hermitian,
+    anticommute, square to 1, and anticommute with gamma_4 = diag(1,1,-1,-1)."""
+    g4 = np.diag([1, 1, -1, -1]).astype(complex)
+    for a in range(3):
+        assert np.allclose(GAMMA[a], GAMMA[a].conj().T)
+        assert np.allclose(GAMMA[a] @ GAMMA[a], np.eye(4))
+        assert np.allclose(GAMMA[a] @ g4 + g4 @ GAMMA[a], 0)
+        for b in range(a + 1, 3):
+            assert np.allclose(GAMMA[a] @ GAMMA[b] + GAMMA[b] @ GAMMA[a], 0)
+
+
+@pytest.mark.parametrize("L,Lt,Ls,am3", [(4, 6, 4, 1.0), (3, 4, 5, 1.3), (6, 6, 8, 1.0)])
+def test_free_field_momentum_space(L, Lt, Ls, am3):
+    """phi = 0, hpin = 0: M is diagonal in momentum (p_x, p_y = 2 pi n / L,
+    p_t = (2n+1) pi / Lt) with the Ls x Ls block
+    (4 - M5 - sum cos p + i sum gamma sin p) delta_ss' - P- delta_s',s+1 - P+ delta_s',s-1."""
+    lat = orc.Lattice(L, Ls, Lt, am3=am3)
+    rs = np.random.RandomState(L * 100 + Ls)
+    R = spinor(rs, lat.shape)
+    chi = np.zeros((3, lat.kvol))
+    out = orc.dslash(lat, R, chi, 0.0, 1, False).reshape(lat.shape)
+    # APBC -> periodic: phi(t) = psi(t) exp(-i pi t / Lt), p_t shifted by pi / Lt
+    t = np.arange(Lt)
+    tw = np.exp(-1j * np.pi * t / Lt)[None, None, :, None, None, None]
+    Rk = np.fft.fftn(R * tw, axes=(2, 3, 4))
+    px = 2 * np.pi * np.fft.fftfreq(L) * 1.0
+    pt = 2 * np.pi * np.fft.fftfreq(Lt) + np.pi / Lt
+    Pm, Pp = np.diag([0, 0, 1, 1]).astype(complex), np.diag([1, 1, 0, 0]).astype(complex)
+    Sp = np.eye(Ls, k=1)   # (S psi)(s) = psi(s+1)
+    Sm = np.eye(Ls, k=-1)  # psi(s-1)
+    Ok = np.zeros_like(Rk)
+    for a in range(Lt):
+        for b in range(L):
+            for c in range(L):
+                p = (px[c], px[b], pt[a])  # x fastest in the array is the last axis
+                W = (4 - am3 - sum(np.cos(q) for q in p)) * np.eye(4)
+                # fourier convention: psi(x+mu) -> exp(+i p) under fftn with exp(-i p x)
+                W = W + 1j * sum(GAMMA[m] * np.sin(p[m]) for m in range(3))
+                blk = np.kron(W, np.eye(Ls)) - np.kron(Pm, Sp) - np.kron(Pp, Sm)
+                for f in range(2):
+                    v = Rk[f, :, a, b, c, :].reshape(4 * Ls)
+                    Ok[f, :, a, b, c, :] = (blk @ v).reshape(4, Ls)
+    expect = np.fft.ifftn(Ok, axes=(2, 3, 4)) / tw
+    assert np.abs(out - expect).max() <= 1e-12 * np.abs(expect).max()
+
+
+@pytest.mark.parametrize("L,Lt,Ls", [(2, 2, 2), (4, 4, 4), (3, 5, 6)])
+@pytest.mark.parametrize("imass", [1, 3])
+def test_adjoint(L, Lt, Ls, imass):
+    lat = orc.Lattice(L, Ls, Lt)
+    rs = np.random.RandomState(9)
+    X, Y, chi = spinor(rs, lat.shape), spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    lhs = np.vdot(Y, orc.dslash(lat, X, chi, 0.4, imass, False))
+    rhs = np.vdot(orc.dslash(lat, Y, chi, 0.4, imass, True), X)
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+
+
+def dense(lat, chi, hpin=0.0, imass=1, dagger=False):
+    n = lat.kferm
+    M = np.zeros((n, n), dtype=complex)
+    e = np.zeros(n, dtype=complex)
+    for j in range(n):
+        e[:] = 0
+        e[j] = 1
+        M[:, j] = orc.dslash(lat, e, chi, hpin, imass, dagger).reshape(-1)
+    return M
+
+
+def test_wall_coupling_structure():
+    """M(phi) - M(0) is nonzero only in the rows s = 1 (P+ components) and
+    s = Ls (P- components), is linear in phi, and the wall blocks are
+    phi.sigma (imass=1) and -/+ i phi.sigma (imass=3)."""
+    lat = orc.Lattice(2, 3, 2)
+    rs = np.random.RandomState(2)
+    chi = phi_field(rs, lat.kvol)
+    M0 = dense(lat, np.zeros_like(chi))
+    for imass in (1, 3):
+        D1 = dense(lat, chi, 0.0, imass) - M0
+        D2 = dense(lat, 2 * chi, 0.0, imass) - M0
+        assert np.allclose(D2, 2 * D1, atol=1e-13)
+        rows = np.argwhere(np.abs(D1).sum(axis=1) > 0)[:, 0]
+        sh = lat.shape
+        for r in rows:
+            f, d, t, y, x, s = np.unravel_index(r, sh)
+            assert (s == 0 and d < 2) or (s == sh[-1] - 1 and d >= 2)
+    # imass=1 block at s=Ls, d=3 (P-), site 0, from s=1
+    D = (dense(lat, chi, 0.0, 1) - M0).reshape(sh + sh)
+    c1, c2, c3 = chi[:, 0]
+    Y = np.array([[c3, c1 - 1j * c2], [c1 + 1j * c2, -c3]])
+    blk = D[:, 2, 0, 0, 0, -1, :, 2, 0, 0, 0, 0]
+    assert np.allclose(blk, Y, atol=1e-14)
+    D3 = (dense(lat, chi, 0.0, 3) - M0).reshape(sh + sh)
+    assert np.allclose(D3[:, 2, 0, 0, 0, -1, :, 2, 0, 0, 0, 0], -1j * Y, atol=1e-14)
+    assert np.allclose(D3[:, 0, 0, 0, 0, 0, :, 0, 0, 0, 0, -1], 1j * Y, atol=1e-14)
+
+
+def test_hpin_only_at_site_one():
+    lat = orc.Lattice(2, 2, 2)
+    chi = np.zeros((3, lat.kvol))
+    D = (dense(lat, chi, 0.9, 1) - dense(lat, chi, 0.0, 1)).reshape(lat.shape + lat.shape)
+    nz = np.argwhere(np.abs(D) > 0)
+    assert len(nz) > 0
+    for ix in nz:
+        assert tuple(ix[2:5]) == (0, 0, 0) and tuple(ix[8:11]) == (0, 0, 0)
+    assert np.isclose(D[0, 2, 0, 0, 0, -1, 0, 2, 0, 0, 0, 0], 0.9)
+    assert np.isclose(D[1, 2, 0, 0, 0, -1, 1, 2, 0, 0, 0, 0], -0.9)
+    assert np.abs(dense(lat, chi, 0.9, 3) - dense(lat, chi, 0.0, 3)).max() == 0
+
+
+def test_qmrherm_vs_dense_rational():
+    lat = orc.Lattice(2, 4, 2)
+    rs = np.random.RandomState(12)
+    chi, phi = phi_field(rs, lat.kvol), spinor(rs, lat.shape)
+    M = dense(lat, chi, 0.3, 1)
+    A = M.conj().T @ M
+    w, U = np.linalg.eigh(A)
+    assert w.min() > 0
+    v = U.conj().T @ phi.reshape(-1)
+    for (a, ad, b, bd), p in ((R4, 0.25), (R2, 0.5)):
+        x, _ = orc.qmrherm(lat, phi, 1e-12, a, ad, chi, 0.3, 1, 0)
+        rat = remez.evaluate((a, ad), w)
+        exact = U @ (rat * v)
+        assert np.abs(x.reshape(-1) - exact).max() <= 1e-10 * np.abs(exact).max()
+        xb, _ = orc.qmrherm(lat, phi, 1e-12, b, bd, chi, 0.3, 1, 0)
+        assert np.abs(xb.reshape(-1) - U @ (remez.evaluate((b, bd), w) * v)).max() <= 1e-9 * np.abs(xb).max()
+
+
+def test_congrad_vs_dense_solve_and_iteration_cap():
+    """CGNR converges to the dense solution; the reference caps it at
+    Ls*kvol iterations (rhmc/cheisdwf.f:1067), which on a 2x2x2x4 lattice
+    (cap 32) stops it early."""
+    lat = orc.Lattice(2, 16, 4)
+    rs = np.random.RandomState(14)
+    chi, phi = phi_field(rs, lat.kvol), spinor(rs, lat.shape)
+    M = dense(lat, chi, 0.3, 1)
+    A = M.conj().T @ M
+    x, it = orc.congrad(lat, phi, 1e-12, chi, None, 0.3, 1)
+    assert it < lat.Ls * lat.kvol
+    exact = np.linalg.solve(A, phi.reshape(-1))
+    assert np.abs(x.reshape(-1) - exact).max() <= 1e-9 * np.abs(exact).max()
+    small = orc.Lattice(2, 4, 2)
+    chi2, phi2 = phi_field(rs, small.kvol), spinor(rs, small.shape)
+    _, it2 = orc.congrad(small, phi2, 1e-12, chi2, None, 0.3, 1)
+    assert it2 == small.Ls * small.kvol
+
+
+def test_force_is_gradient_of_action():
+    """Central finite difference of S(chi) = beta sum chi^2 + Phi^dagger
+    r(M^dagger M) Phi, with r the remez2g b-set the force's guidance solve
+    uses, against the force of orc_force (remez2g a-set; the two 12-pole sets
+    are accurate to 7e-7, which bounds the agreement)."""
+    lat = orc.Lattice(2, 4, 2)
+    rs = np.random.RandomState(13)
+    chi, phi = phi_field(rs, lat.kvol), spinor(rs, lat.shape)
+    beta = 0.475
+    F, _, _ = orc.force(lat, phi, 1e-13, chi, beta, R2G)
+    a, ad, b, bd = R2G
+
+    def S(c):
+        x, _ = orc.qmrherm(lat, phi, 1e-13, b, bd, c, 0.0, 1, 0)
+        return beta * np.sum(c * c) + np.vdot(phi, x).real
+
+    eps = 1e-5
+    for comp in range(3):
+        for site in (0, 3, lat.kvol - 1):
+            cp, cm = chi.copy(), chi.copy()
+            cp[comp, site] += eps
+            cm[comp, site] -= eps
+            fd = (S(cp) - S(cm)) / (2 * eps)
+            assert abs(fd - F[comp, site]) <= 2e-5 * max(1.0, abs(fd)), (comp, site, fd, F[comp, site])
+
+
+def test_rng_closed_form():
+    """s_n = s_0 * M^n mod 2^48 with M = 2651554 * 2^24 + 15184245."""
+    Mul, mod = 2651554 * 2 ** 24 + 15184245, 2 ** 48
+    seed = 693764523745
+    r = orc.Rng(float(seed))
+    for n in range(1, 300):
+        u = r.rranf()
+        assert u == (seed * pow(Mul, n, mod) % mod) / 2.0 ** 48
+    r = orc.Rng(float(seed))
+    z = r.gauss0(200000)
+    assert abs(np.mean(z[0] ** 2 + z[1] ** 2) - 1.0) < 0.01   # E|z|^2 = 1
+    g = r.gaussp(200000)
+    assert abs(np.var(g[0]) - 1.0) < 0.02 and abs(np.mean(g[1])) < 0.01
+
+
+@pytest.mark.parametrize("name,p,tol", [("remez4", 0.25, 1e-11), ("remez2", 0.5, 1e-11), ("remez2g", 0.5, 2e-6)])
+def test_remez_sets_approximate_powers(name, p, tol):
+    a, ad, b, bd = remez.load(name)
+    x = np.logspace(-3.9, np.log10(45.0), 400)
+    ra = remez.evaluate((a, ad), x)
+    rb = remez.evaluate((b, bd), x)
+    assert np.max(np.abs(ra / x ** p - 1)) <= tol
+    assert np.max(np.abs(rb * x ** p - 1)) <= tol
