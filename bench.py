@@ -38,6 +38,20 @@ def make_inputs(lat, beta, seed=693764523745.0):
     return chi, b
 
 
+def measured_traffic(L, Ls):
+    """HBM bytes per launch of the roofline kernel from the FETCH_SIZE /
+    WRITE_SIZE counter runs recorded in profiles/ (scaled by a plain copy of
+    known size), for the workload they were measured on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "dslash_traffic.json")
+    try:
+        rec = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if rec.get("L") != L or rec.get("Ls") != Ls:
+        return None
+    return rec["traffic_bytes"]
+
+
 def cpu_baseline(L, Ls, beta, fp64_iters, budget_s=20.0):
     """Bounded sample of the CPU oracle (C/OpenMP, FP64, all host cores):
     a few CGNR iterations of the same system, scaled by the FP64 GPU solve's
@@ -74,7 +88,9 @@ def main():
     ap.add_argument("--beta", type=float, default=0.475)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--form", default="fp32", choices=["fp64", "fp32", "bf16"])
+    ap.add_argument("--delta", type=float, default=0.0, help="reliable-update threshold (0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-dslash", action="store_true", help="only the roofline kernel and a calibration copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box of this deployment has one MI355X, and the path "
@@ -83,6 +99,17 @@ def main():
 
     from chiralheisenberg_amd import fermion as fm
     lat = fm.Lattice(args.L, args.Ls)
+    if args.only_dslash:
+        # counter runs: the roofline kernel plus one plain copy of a spinor
+        # (k_convert<F64,F64>: 16 B read + 16 B written per complex number),
+        # which calibrates FETCH_SIZE / WRITE_SIZE against known bytes
+        a = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+        b2 = fm.spinor_dev(lat, None)
+        fm.check(lat.h, fm.lib().chs_convert_dev(lat.h, b2.ptr, 0, a.ptr, 0, lat.kferm))
+        lat.synchronize()
+        print("bench: dslash fp64 %.2f us" % (fm.time_dslash(lat, "fp64", False, nbuf=4, iters=20) * 1e3),
+              flush=True)
+        return
     chi, b = make_inputs(lat, args.beta)
     chi_d = fm.DeviceArray.from_numpy(lat, chi)
     b_d = fm.spinor_dev(lat, b)
@@ -92,7 +119,7 @@ def main():
     def solve(form):
         x_d.upload(zero)
         lat.synchronize()
-        return fm.solve_mdagm(lat, b_d, x_d, chi_d, args.tol, form)
+        return fm.solve_mdagm(lat, b_d, x_d, chi_d, args.tol, form, delta=args.delta if form != "fp64" else 0.0)
 
     print("bench: L=%d Ls=%d form=%s warmup=%d steps=%d" % (args.L, args.Ls, args.form, args.warmup, args.steps),
           file=sys.stderr, flush=True)
@@ -147,7 +174,7 @@ def main():
         "speedup_vs_fp64": st64["seconds"] / max(wall, 1e-12),
         "dslash_us": {"fp64": ms * 1e3, "fp32": ms32 * 1e3, "bf16": ms16 * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(args.L, args.Ls),
                      "kernel": "k_dsl<F64,F64,false,EPI_STORE>", "bytes_per_launch": 2 * V + C},
     }
     if not args.no_cpu:

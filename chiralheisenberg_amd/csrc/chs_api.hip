@@ -780,7 +780,7 @@ int chs_solve_mdagm_dev(chs_ctx *c, const double *b, double *x, const double *ch
         return rcg_solve<F64, F64>(c, b, x, chi, hpin, imass, tol, delta, maxiter, stats);
     }
     if (prec == CHS_FP32) {
-        if (delta <= 0) delta = 1e-3;
+        if (delta <= 0) delta = 0.1;
         return rcg_solve<F32, F32>(c, b, x, chi, hpin, imass, tol, delta, maxiter, stats);
     }
     if (prec == CHS_BF16) {
