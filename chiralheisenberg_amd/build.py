@@ -45,16 +45,37 @@ def build_lib(verbose=False):
     return LIB
 
 
-def build_fortran_shim(outdir, flang="/opt/rocm/llvm/bin/flang", defines=None):
-    """Compile the iso_c_binding shim (chs_shim.F90) into an object in outdir."""
+FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")
+SHIM_SIZES = (8, 8, 8)   # ksize, ksizet, kthird of the shim library the tests drive
+
+
+def shim_lib_path(sizes=SHIM_SIZES):
+    return os.path.join(HERE, "fortran", "libchs_shim_l%d_lt%d_ls%d.so" % sizes)
+
+
+def build_fortran_shim(sizes=SHIM_SIZES, verbose=False):
+    """Compile the Fortran drop-in binding (fortran/chs_shim.F90) for the
+    given compile-time sizes, with the reference's FP64 default kinds
+    (-fdefault-real-8), into a shared library linked against libchs.so.
+    Returns its path, or None when no flang is available."""
+    if not os.path.exists(FLANG):
+        return None
     src = os.path.join(HERE, "fortran", "chs_shim.F90")
-    os.makedirs(outdir, exist_ok=True)
-    obj = os.path.join(outdir, "chs_shim.o")
-    cmd = [flang, "-O2", "-fPIC", "-c", src, "-o", obj, "-module-dir", outdir]
-    for k, v in (defines or {}).items():
-        cmd.append("-D%s=%s" % (k, v))
-    subprocess.run(cmd, check=True, capture_output=True, text=True)
-    return obj
+    bdir = os.path.join(HERE, "_build", "shim_l%d_lt%d_ls%d" % sizes)
+    os.makedirs(bdir, exist_ok=True)
+    obj = os.path.join(bdir, "chs_shim.o")
+    out = shim_lib_path(sizes)
+    cmds = [[FLANG, "-cpp", "-O2", "-fPIC", "-fdefault-real-8", "-fdefault-double-8",
+             "-DKSIZE=%d" % sizes[0], "-DKSIZET=%d" % sizes[1], "-DKTHIRD=%d" % sizes[2],
+             "-module-dir", bdir, "-c", src, "-o", obj],
+            [FLANG, "-shared", "-o", out, obj, "-L" + HERE, "-lchs", "-Wl,-rpath,$ORIGIN/.."]]
+    for c in cmds:
+        if verbose:
+            print(" ".join(c), flush=True)
+        res = subprocess.run(c, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError("flang failed:\n" + res.stderr[-4000:])
+    return out
 
 
 if __name__ == "__main__":

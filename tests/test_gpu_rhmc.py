@@ -1,0 +1,47 @@
+"""RHMC trajectories (program dwf3d, rhmc/cheisdwf.f:207-413) with every
+fermion routine on the GPU against the oracle's own trajectory loop, from
+the same seed and the same chi: identical trajectory lengths and
+accept/reject decisions, and H0, H1, dH in agreement.  The solves run at
+tight tolerances (1e-11 / 1e-10) so that the comparison is not limited by
+the solver tolerance; the reference's production tolerances (1e-9, 1e-6)
+are the defaults of both drivers."""
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+fm = pytest.importorskip("chiralheisenberg_amd.fermion")
+from chiralheisenberg_amd import remez, rhmc  # noqa: E402
+
+
+@pytest.mark.parametrize("L,Ls,ntraj,imass,dt", [(4, 4, 8, 1, 0.45), (4, 6, 4, 3, 0.3)])
+def test_trajectories_match_oracle(L, Ls, ntraj, imass, dt):
+    """dt = 0.45 at L=4, Ls=4 gives a mix of accepted and rejected proposals."""
+    par = dict(dt=dt, beta=0.475, hpin=0.0, imass=imass, iterl=3, itermax=1000, iprint=2,
+               rescga=1e-11, rescgg=1e-10, am=0.0)
+    sets = {k: remez.load(k) for k in ("remez2", "remez4", "remez2g")}
+    rs = np.random.RandomState(100 + L)
+    chi0 = rs.randn(3, L * L * L) / np.sqrt(0.475)
+    seed = 693764523745.0
+
+    lat = fm.Lattice(L, Ls)
+    chi_g = chi0.copy()
+    rng = rhmc.RandomStream(seed)
+    gpu = [rhmc.trajectory(lat, chi_g, rng, isweep=i + 1, par=par, sets=sets) for i in range(ntraj)]
+
+    olat = orc.Lattice(L, Ls)
+    chi_c = chi0.copy()
+    orng = orc.Rng(seed)
+    cpu = orc.rhmc(olat, par, orng, chi_c, 1, ntraj, sets)
+
+    assert [g["md_steps"] for g in gpu] == [c["md_steps"] for c in cpu]
+    assert [g["accepted"] for g in gpu] == [c["accepted"] for c in cpu]
+    for g, c in zip(gpu, cpu):
+        for k in ("H0", "H1"):
+            assert abs(g[k] - c[k]) <= 1e-10 * abs(c[k]), (k, g[k], c[k])
+        assert abs(g["dH"] - c["dH"]) <= 1e-8 * max(1.0, abs(c["dH"])), (g["dH"], c["dH"])
+    assert np.abs(chi_g - chi_c).max() <= 1e-9 * np.abs(chi_c).max()
+    assert rng.seed() == orng.get()
+    lat.close()
