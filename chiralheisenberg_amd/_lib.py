@@ -36,6 +36,23 @@ class Rng(ctypes.Structure):
                 ("yran", ctypes.c_double), ("yrun", ctypes.c_double)]
 
 
+class MsSystem(ctypes.Structure):
+    _fields_ = [("ctx", ctypes.c_void_p), ("phi", ctypes.c_void_p), ("chi", ctypes.c_void_p), ("x", ctypes.c_void_p),
+                ("hpin", ctypes.c_double), ("imass", ctypes.c_int), ("ndiag", ctypes.c_int),
+                ("anum", ctypes.c_void_p), ("aden", ctypes.c_void_p), ("res", ctypes.c_double),
+                ("itercg", ctypes.c_int)]
+
+
+class BatchTune(ctypes.Structure):
+    _fields_ = [("split", ctypes.c_int), ("chunk", ctypes.c_int), ("graph", ctypes.c_int),
+                ("us_per_iter", ctypes.c_double)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("graph_replays", ctypes.c_int), ("launches", ctypes.c_int),
+                ("seconds", ctypes.c_double), ("tune", BatchTune)]
+
+
 def _declare(L):
     c, i, d, l, v, sz = ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_long, ctypes.c_void_p, ctypes.c_size_t
     ip = ctypes.POINTER(ctypes.c_int)
@@ -66,6 +83,19 @@ def _declare(L):
         "chs_congrad_dev": ([v, v, d, ip, d, i, v, v], i),
         "chs_solve_mdagm_dev": ([v, v, v, v, d, i, d, i, d, i, ctypes.POINTER(SolveStats)], i),
         "chs_time_dslash": ([v, i, i, i, i, d, i, ctypes.POINTER(ctypes.c_double)], i),
+        "chs_nn_init": ([v, i, ctypes.c_ulonglong], i),
+        "chs_nn_num_weights": ([v], l),
+        "chs_nn_set_weights": ([v, v, l], i),
+        "chs_nn_get_weights": ([v, v, l], i),
+        "chs_nn_apply_dev": ([v, v, v], i),
+        "chs_nn_loss_grad_dev": ([v, v, v, ctypes.POINTER(ctypes.c_double), v], i),
+        "chs_nn_train_dev": ([v, v, v, i, i, ctypes.c_float, v], i),
+        "chs_solve_mdagm_fcg_dev": ([v, v, v, v, d, i, d, d, i, i, ctypes.POINTER(SolveStats)], i),
+        "chs_batch_create": ([i], v),
+        "chs_batch_destroy": ([v], None),
+        "chs_batch_last_error": ([v], c),
+        "chs_qmrherm_batch_dev": ([v, ctypes.POINTER(MsSystem), i, ctypes.POINTER(BatchTune),
+                                   ctypes.POINTER(BatchStats)], i),
         "chs_rng_set": ([ctypes.POINTER(Rng), d], None),
         "chs_rng_get": ([ctypes.POINTER(Rng)], d),
         "chs_rano": ([ctypes.POINTER(Rng), i], d),

@@ -1,0 +1,70 @@
+"""Batched multi-shift solves (host side, over libchs's chs_qmrherm_batch_dev).
+
+Many qmrherm solves (rhmc/cheisdwf.f:599-745 with iflag = 0) -- several
+pseudofermion fields, lattices of different sizes -- run in lock step, one
+grouped launch per Lanczos stage, chunks of steps replayed from hipGraphs,
+launch parameters chosen by the library's autotuner.  Each result equals the
+one-at-a-time chs_qmrherm_dev result bit for bit."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import ChsError, lib
+
+
+class System:
+    """One solve: lat (fermion.Lattice), phi / chi / x device arrays (FP64
+    native layout), remez coefficients (anum, aden) and qmrherm's res."""
+
+    def __init__(self, lat, phi, chi, x, anum, aden, res, hpin=0.0, imass=1):
+        self.lat, self.phi, self.chi, self.x = lat, phi, chi, x
+        self.anum = np.ascontiguousarray(anum, dtype=np.float64)
+        self.aden = np.ascontiguousarray(aden, dtype=np.float64)
+        assert self.anum.size == self.aden.size + 1
+        self.res, self.hpin, self.imass = float(res), float(hpin), int(imass)
+        self.itercg = None
+
+
+class Batch:
+    def __init__(self, device=0):
+        self.h = lib().chs_batch_create(device)
+        if not self.h:
+            raise ChsError(_lib.CHS_EHIP, lib().chs_create_error().decode())
+
+    def _check(self, rc):
+        if rc != _lib.CHS_OK:
+            raise ChsError(rc, lib().chs_batch_last_error(self.h).decode())
+
+    def qmrherm(self, systems, tune=None):
+        """Solve every system; sets System.itercg and returns the batch
+        statistics (dict; 'tune' holds the launch parameters used)."""
+        arr = (_lib.MsSystem * len(systems))()
+        for a, s in zip(arr, systems):
+            a.ctx, a.phi, a.chi, a.x = s.lat.h, s.phi.ptr, s.chi.ptr, s.x.ptr
+            a.hpin, a.imass, a.ndiag, a.res = s.hpin, s.imass, s.aden.size, s.res
+            a.anum = s.anum.ctypes.data_as(ctypes.c_void_p)
+            a.aden = s.aden.ctypes.data_as(ctypes.c_void_p)
+        t = None
+        if tune is not None:
+            t = _lib.BatchTune(int(tune["split"]), int(tune["chunk"]), int(tune["graph"]), 0.0)
+        st = _lib.BatchStats()
+        self._check(lib().chs_qmrherm_batch_dev(self.h, arr, len(systems), ctypes.byref(t) if t else None,
+                                                 ctypes.byref(st)))
+        for a, s in zip(arr, systems):
+            s.itercg = a.itercg
+        tu = st.tune
+        return {"iterations": st.iterations, "graph_replays": st.graph_replays, "launches": st.launches,
+                "seconds": st.seconds,
+                "tune": {"split": tu.split, "chunk": tu.chunk, "graph": tu.graph, "us_per_iter": tu.us_per_iter}}
+
+    def close(self):
+        if self.h:
+            lib().chs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -41,7 +41,9 @@ struct chs_ctx {
     double *dcoef = nullptr;      // remez coefficients (anum) on the device
     void *hpinned = nullptr;      // pinned host scratch for polling
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    struct NNState *nn = nullptr; // learned preconditioner (chs_learn_host.inc)
 };
+static void nn_destroy(chs_ctx *c);
 
 static thread_local std::string g_create_err;
 
@@ -265,6 +267,7 @@ void chs_ctx_destroy(chs_ctx *c) {
     if (c->hpinned) hipHostFree(c->hpinned);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    nn_destroy(c);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -820,3 +823,7 @@ int chs_time_dslash(chs_ctx *c, int prec, int dagger, int nbuf, int iters, doubl
 }
 
 }  // extern "C"
+
+// learned preconditioner and flexible CG (same translation unit)
+#include "chs_learn_host.inc"
+#include "chs_batch_host.inc"

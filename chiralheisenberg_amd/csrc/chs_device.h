@@ -127,8 +127,9 @@ enum Epi {
     EPI_NORM = 1,    // out = M in, partial += |out|^2
     EPI_SCALEQ = 2,  // in = r: out = (M r)/beta, aux0 = q = r/beta (Lanczos, scal[0] = beta)
     EPI_DOT = 3,     // out = M in, partial += Re <aux0, out>
-    EPI_CG = 4       // no store: aux1 (x) += alpha aux0 (p), aux2 (r) -= alpha (M in),
+    EPI_CG = 4,      // no store: aux1 (x) += alpha aux0 (p), aux2 (r) -= alpha (M in),
                      // partial += |r|^2; alpha = scal[0]; flag bit0: skip x update
+    EPI_CGQ = 5      // EPI_CG that also stores out = M in (flexible CG keeps A p)
 };
 
 struct EpiArgs {
@@ -226,8 +227,7 @@ __global__ void k_lz_init(LzState *st, const double *__restrict__ partial, int n
 
 // alphatild = Re <q, M^dagger M q>; then the shift recurrences of one step
 // (rhmc/cheisdwf.f:666-687), which use the previous beta.
-__global__ void k_lz_alpha(LzState *st, const double *__restrict__ partial, int np, int iter) {
-    if (st->done) return;
+__device__ __forceinline__ void lz_alpha(LzState *st, const double *__restrict__ partial, int np, int iter) {
     const double at = sum_partials(partial, np);
     __shared__ double sat;
     if (threadIdx.x == 0) {
@@ -253,10 +253,14 @@ __global__ void k_lz_alpha(LzState *st, const double *__restrict__ partial, int 
     }
 }
 
+__global__ void k_lz_alpha(LzState *st, const double *__restrict__ partial, int np, int iter) {
+    if (st->done) return;
+    lz_alpha(st, partial, np, iter);
+}
+
 // new beta = |r|; convergence max_k |phimod rho_k| < 2 resid for iter >= 2
 // (rhmc/cheisdwf.f:697-715).
-__global__ void k_lz_beta(LzState *st, const double *__restrict__ partial, int np, int iter) {
-    if (st->done) return;
+__device__ __forceinline__ void lz_beta(LzState *st, const double *__restrict__ partial, int np, int iter) {
     const double t = sum_partials(partial, np);
     if (threadIdx.x == 0) {
         st->beta = sqrt(t);
@@ -272,24 +276,26 @@ __global__ void k_lz_beta(LzState *st, const double *__restrict__ partial, int n
     }
 }
 
+__global__ void k_lz_beta(LzState *st, const double *__restrict__ partial, int np, int iter) {
+    if (st->done) return;
+    lz_beta(st, partial, np, iter);
+}
+
 // r = w - alphatild q - beta0 qm1 (|r|^2 partial) and, for every shift,
 // p_k = q - mu_k p_k, x_k += rho_k p_k (iter 1: p_k = q, x_k = rho_k q).
 // p and x are shift-major: shift k at offset k*n.
 template <int NS>
-__global__ __launch_bounds__(kBlock) void k_lz_update(long n, const double2 *__restrict__ w,
-                                                      const double2 *__restrict__ q,
-                                                      const double2 *__restrict__ qm1, double2 *__restrict__ r,
-                                                      double2 *__restrict__ p, double2 *__restrict__ x,
-                                                      const LzState *__restrict__ st, int ndiag, int iter,
-                                                      double *__restrict__ partial) {
-    if (st->done) return;
+__device__ __forceinline__ void lz_update(long n, const double2 *__restrict__ w, const double2 *__restrict__ q,
+                                          const double2 *__restrict__ qm1, double2 *__restrict__ r,
+                                          double2 *__restrict__ p, double2 *__restrict__ x,
+                                          const LzState *__restrict__ st, int ndiag, int iter,
+                                          double *__restrict__ partial, long lb) {
     __shared__ double s_amu[kMaxShift], s_rho[kMaxShift];
     if (threadIdx.x < ndiag) {
         s_amu[threadIdx.x] = st->amu[threadIdx.x];
         s_rho[threadIdx.x] = st->rho[threadIdx.x];
     }
     __syncthreads();
-    const long lb = logical_block();
     const long i = lb * kBlock + threadIdx.x;
     double part = 0.0;
     if (i < n) {
@@ -324,6 +330,17 @@ __global__ __launch_bounds__(kBlock) void k_lz_update(long n, const double2 *__r
     }
     const double t = block_sum(part);
     if (threadIdx.x == 0) partial[lb] = t;
+}
+
+template <int NS>
+__global__ __launch_bounds__(kBlock) void k_lz_update(long n, const double2 *__restrict__ w,
+                                                      const double2 *__restrict__ q,
+                                                      const double2 *__restrict__ qm1, double2 *__restrict__ r,
+                                                      double2 *__restrict__ p, double2 *__restrict__ x,
+                                                      const LzState *__restrict__ st, int ndiag, int iter,
+                                                      double *__restrict__ partial) {
+    if (st->done) return;
+    lz_update<NS>(n, w, q, qm1, r, p, x, st, ndiag, iter, partial, logical_block());
 }
 
 // x = a0 phi + sum_k a_k x_k, summed in shift order (rhmc/cheisdwf.f:722-727)

@@ -194,15 +194,15 @@ __host__ __device__ inline unsigned stencil_blocks_per_flavour(long n5) {
 // out = M in / M^dagger in with the epilogues of chs::Epi.  1-D grid of
 // 2 * blocks_per_flavour workgroups rounded up to a multiple of 8; logical
 // block lb covers flavour lb / bpf, elements (lb % bpf) * kBlock + tid.
-// Partials: one per workgroup at partial[logical block].
-template <class S, class SA, bool DAG, int EPI, int LAYOUT = 1, int WPE = 1>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_dsl(
-    Lat lat, typename S::st *__restrict__ out, const typename S::st *__restrict__ in, const double *__restrict__ chi,
-    double hpin, int imass, EpiArgs ea) {
+// Partials: one per workgroup at partial[logical block].  dsl_block is the
+// work of one logical block; k_dsl launches it for one lattice and the
+// grouped kernels of chs_batch.h for many.
+template <class S, class SA, bool DAG, int EPI, int LAYOUT>
+__device__ __forceinline__ void dsl_block(const Lat &lat, typename S::st *__restrict__ out,
+                                          const typename S::st *__restrict__ in, const double *__restrict__ chi,
+                                          double hpin, int imass, const EpiArgs &ea, unsigned lb) {
     typedef typename S::rt T;
     typedef typename SA::rt TA;
-    if (ea.skip && *ea.skip) return;
-    const unsigned lb = logical_block_x();
     const unsigned bpf = stencil_blocks_per_flavour(lat.n5);
     const int fb = (int)(lb / bpf);
     const unsigned g = (lb - (unsigned)fb * bpf) * kBlock + threadIdx.x;   // position in (s, site) order
@@ -228,7 +228,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
                 const T ib = (T)(1.0 / ea.scal[0]);
                 store_nt<S>(out + po, g, cscale(ib, o[0][d]));
                 store_nt<S>((typename S::st *)ea.aux0 + po, g, cscale(ib, c[0][d]));
-            } else if (EPI == EPI_CG) {
+            } else if (EPI == EPI_CG || EPI == EPI_CGQ) {
+                if (EPI == EPI_CGQ) store_nt<S>(out + po, g, o[0][d]);
                 const TA alpha = (TA)ea.scal[0];
                 typename SA::st *xr = (typename SA::st *)ea.aux1 + po;
                 typename SA::st *rr = (typename SA::st *)ea.aux2 + po;
@@ -253,10 +254,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
         }
         part = (double)acc;
     }
-    if (EPI == EPI_NORM || EPI == EPI_DOT || EPI == EPI_CG) {
+    if (EPI == EPI_NORM || EPI == EPI_DOT || EPI == EPI_CG || EPI == EPI_CGQ) {
         const double t = block_sum(part);
         if (threadIdx.x == 0) ea.partial[lb] = t;
     }
+}
+
+template <class S, class SA, bool DAG, int EPI, int LAYOUT = 1, int WPE = 1>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_dsl(
+    Lat lat, typename S::st *__restrict__ out, const typename S::st *__restrict__ in, const double *__restrict__ chi,
+    double hpin, int imass, EpiArgs ea) {
+    if (ea.skip && *ea.skip) return;
+    dsl_block<S, SA, DAG, EPI, LAYOUT>(lat, out, in, chi, hpin, imass, ea, logical_block_x());
 }
 
 // Grid size (workgroups) of a k_dsl launch.

@@ -1,0 +1,97 @@
+"""Learned preconditioner for M^dagger M solves (host side, over libchs).
+
+The network lives on the GPU (chs_nn_* entry points): a 4-layer 3-D
+convolutional net over (x, y, t) on the Ls-stacked spinor, BF16 MFMA kernels
+for inference and training.  Training data are (source, solution) pairs of
+M^dagger M x = b produced by the plain solver; the trained network is the
+preconditioner of flexible CG, whose answer is judged by the FP64 true
+residual like every other solve (plain CG remains the fallback).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from . import fermion as fm
+
+
+class Preconditioner:
+    def __init__(self, lat, hidden=16, seed=1234):
+        self.lat = lat
+        check(lat.h, lib().chs_nn_init(lat.h, hidden, seed))
+        self.nweights = lib().chs_nn_num_weights(lat.h)
+
+    def get_weights(self):
+        w = np.empty(self.nweights, dtype=np.float32)
+        check(self.lat.h, lib().chs_nn_get_weights(self.lat.h, w.ctypes.data_as(ctypes.c_void_p), self.nweights))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.nweights
+        check(self.lat.h, lib().chs_nn_set_weights(self.lat.h, w.ctypes.data_as(ctypes.c_void_p), self.nweights))
+
+    def apply(self, r_dev, z_dev):
+        """z = N(r) on FP32 native device spinors."""
+        check(self.lat.h, lib().chs_nn_apply_dev(self.lat.h, r_dev.ptr, z_dev.ptr))
+
+    def loss_grad(self, src_dev, sol_dev):
+        loss = ctypes.c_double()
+        g = np.empty(self.nweights, dtype=np.float32)
+        check(self.lat.h, lib().chs_nn_loss_grad_dev(self.lat.h, src_dev.ptr, sol_dev.ptr, ctypes.byref(loss),
+                                                      g.ctypes.data_as(ctypes.c_void_p)))
+        return loss.value, g
+
+    def train(self, src_dev, sol_dev, npairs, epochs=1, lr=1e-3):
+        """Adam over npairs pairs stored contiguously (FP32 native spinors);
+        returns the mean loss of each epoch."""
+        out = np.zeros(epochs)
+        check(self.lat.h, lib().chs_nn_train_dev(self.lat.h, src_dev.ptr, sol_dev.ptr, int(npairs), int(epochs),
+                                                  float(lr), out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+
+def solve_fcg(lat, b_dev, x_dev, chi_dev, tol=1e-10, use_nn=True, hpin=0.0, imass=1, delta=0.1, maxiter=0):
+    """Flexible CG on M^dagger M x = b (FP64 native device arrays) to FP64 true
+    relative residual tol."""
+    st = _lib.SolveStats()
+    check(lat.h, lib().chs_solve_mdagm_fcg_dev(lat.h, b_dev.ptr, x_dev.ptr, chi_dev.ptr, hpin, imass, tol, delta,
+                                                maxiter, int(bool(use_nn)), ctypes.byref(st)))
+    return {f: getattr(st, f) for f, _ in _lib.SolveStats._fields_}
+
+
+def pairs_dev(lat, chi_dev, npairs, rng_seed=11, tol=1e-10, precision="fp32", kind="solve", hpin=0.0, imass=1):
+    """Generate npairs (source, solution) pairs of M^dagger M x = b.
+    kind "solve": Gaussian b, x from the plain solver; kind "apply": Gaussian
+    x, b = M^dagger M x (exact, no solve).  Returns (src, sol) FP32 native
+    device arrays (npairs spinors each) and the solver time in seconds."""
+    if kind not in ("solve", "apply"):
+        raise ValueError("kind must be 'solve' or 'apply'")
+    n = lat.kferm
+    src = fm.DeviceArray(lat, npairs * n * 8)
+    sol = fm.DeviceArray(lat, npairs * n * 8)
+    rs = np.random.RandomState(rng_seed)
+    x = fm.spinor_dev(lat, None)
+    t = fm.spinor_dev(lat, None)
+    zero = np.zeros(lat.shape, np.complex128)
+    seconds = 0.0
+    for i in range(npairs):
+        gh = (rs.randn(*lat.shape) + 1j * rs.randn(*lat.shape)) / np.sqrt(2.0)
+        if kind == "solve":
+            bd = fm.spinor_dev(lat, gh)
+            x.upload(zero)
+            st = fm.solve_mdagm(lat, bd, x, chi_dev, tol, precision, hpin=hpin, imass=imass)
+            seconds += st["seconds"]
+        else:
+            bd = fm.spinor_dev(lat, None)
+            x.free()
+            x = fm.spinor_dev(lat, gh)
+            fm.dslash_dev(lat, t, x, chi_dev, hpin, imass, dagger=False)
+            fm.dslash_dev(lat, bd, t, chi_dev, hpin, imass, dagger=True)
+        for dst, srcv in ((src, bd), (sol, x)):
+            check(lat.h, lib().chs_convert_dev(lat.h, ctypes.c_void_p(dst.ptr + i * n * 8), _lib.CHS_FP32, srcv.ptr,
+                                               _lib.CHS_FP64, n))
+        bd.free()
+    lat.synchronize()
+    return src, sol, seconds

@@ -176,6 +176,76 @@ int chs_solve_mdagm_dev(chs_ctx *ctx, const double *b, double *x,
                         int prec, double delta, int maxiter,
                         chs_solve_stats *stats);
 
+/* ---- learned preconditioner (no counterpart in the reference) ------------
+ * A 4-layer 3-D convolutional network (3x3x3 kernels over x, y, t, periodic
+ * padding) on the Ls-stacked spinor: 16*Ls input/output channels
+ * (s, spin-flavour plane, re/im), `hidden` channels between, tanh, no bias,
+ * BF16 MFMA with FP32 accumulation, plus an identity path: with u = r / rms(r)
+ * it returns u + net(u), an approximation of the direction of
+ * (M^dagger M)^{-1} r, and is used inside flexible CG. */
+int chs_nn_init(chs_ctx *ctx, int hidden, unsigned long long seed);
+long chs_nn_num_weights(chs_ctx *ctx);
+/* all weights, FP32, layers concatenated, each W[tap][cout][cin] */
+int chs_nn_set_weights(chs_ctx *ctx, const float *w, long n);
+int chs_nn_get_weights(chs_ctx *ctx, float *w, long n);
+/* z = N(r): FP32 native-layout device spinors */
+int chs_nn_apply_dev(chs_ctx *ctx, const float *r, float *z);
+/* loss (mean squared error of N(src) against sol / rms(sol)) and its gradient
+ * (host array of chs_nn_num_weights floats, may be NULL) for one pair */
+int chs_nn_loss_grad_dev(chs_ctx *ctx, const float *src, const float *sol, double *loss, float *grad);
+/* Adam training over npairs (source, solution) pairs stored contiguously as
+ * FP32 native spinors; epoch_loss (epochs entries) receives the mean loss of
+ * each epoch. */
+int chs_nn_train_dev(chs_ctx *ctx, const float *src, const float *sol, int npairs, int epochs, float lr,
+                     double *epoch_loss);
+/* Flexible CG (FP32 iteration, FP64 reliable updates every delta of
+ * residual reduction, answer judged by the FP64 true relative residual) with
+ * z = N(r) (use_nn = 1) or z = r (use_nn = 0, plain CG). */
+int chs_solve_mdagm_fcg_dev(chs_ctx *ctx, const double *b, double *x, const double *chi, double hpin, int imass,
+                            double tol, double delta, int maxiter, int use_nn, chs_solve_stats *stats);
+
+/* ---- batched multi-shift solves (no counterpart in the reference) ----------
+ * Many qmrherm solves (iflag = 0) at once -- pseudofermion fields, lattices
+ * of different sizes -- iterated in lock step with one grouped launch per
+ * Lanczos stage, a chunk of steps replayed from a hipGraph.  Every system's
+ * result and itercg equal those of chs_qmrherm_dev on the same inputs bit for
+ * bit.  Systems may share a context (its geometry is all that is used); all
+ * must be on the batch's device.  Device arrays in the native layout. */
+typedef struct chs_batch chs_batch;
+typedef struct {
+    chs_ctx *ctx;           /* lattice geometry */
+    const double *phi;      /* source (device, FP64) */
+    const double *chi;      /* auxiliary field (device) */
+    double *x;              /* out: anum[0] phi + sum_k anum[k+1] x_k (device) */
+    double hpin;
+    int imass;
+    int ndiag;
+    const double *anum;     /* host, ndiag + 1 */
+    const double *aden;     /* host, ndiag */
+    double res;             /* qmrherm's res: stop when max_k |rho_k| phimod < 2 sqrt(kferm) res */
+    int itercg;             /* out */
+} chs_ms_system;
+/* launch parameters: split = workgroups sharing one element block of the
+ * shift update (1..8), chunk = iterations per poll (even), graph = replay
+ * chunks from a hipGraph; us_per_iter = the tuner's measurement */
+typedef struct {
+    int split, chunk, graph;
+    double us_per_iter;
+} chs_batch_tune;
+typedef struct {
+    int iterations;         /* lock-step iterations run (>= every itercg) */
+    int graph_replays;
+    int launches;           /* kernel launches enqueued one by one */
+    double seconds;
+    chs_batch_tune tune;    /* the parameters used after tuning */
+} chs_batch_stats;
+chs_batch *chs_batch_create(int device);   /* NULL on failure: chs_create_error() */
+void chs_batch_destroy(chs_batch *b);
+const char *chs_batch_last_error(chs_batch *b);
+/* tune = NULL: autotune on the first chunks (cached per batch shape) */
+int chs_qmrherm_batch_dev(chs_batch *b, chs_ms_system *sys, int nsys, const chs_batch_tune *tune,
+                          chs_batch_stats *stats);
+
 /* Average duration in milliseconds of `iters` back-to-back dslash launches
  * cycling through nbuf (in, out) device buffer pairs, measured with HIP
  * events on the context stream.  Used for the HBM roofline figure. */

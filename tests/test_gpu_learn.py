@@ -1,0 +1,91 @@
+"""The learned preconditioner: GPU network (BF16 MFMA kernels) against the
+CPU oracle oracle/nn_oracle.py (same BF16 rounding points, FP64
+accumulation), flexible CG with and without the network, and training."""
+import numpy as np
+import pytest
+
+import nn_oracle as nno
+from conftest import phi_field, spinor
+
+pytestmark = pytest.mark.gpu
+
+fm = pytest.importorskip("chiralheisenberg_amd.fermion")
+from chiralheisenberg_amd import learn  # noqa: E402
+
+
+def small():
+    lat = fm.Lattice(4, 2, 4)   # L=4, Lt=4, Ls=2: 32 channels, 64 sites
+    rs = np.random.RandomState(5)
+    pc = learn.Preconditioner(lat, hidden=16, seed=7)
+    w = (rs.randn(pc.nweights) * 0.1).astype(np.float32)
+    pc.set_weights(w)
+    return lat, pc, w, rs
+
+
+def test_network_forward_matches_oracle():
+    lat, pc, w, rs = small()
+    src = spinor(rs, lat.shape).astype(np.complex64).astype(np.complex128)
+    r = fm.spinor_dev(lat, src, "fp32")
+    z = fm.spinor_dev(lat, None, "fp32")
+    pc.apply(r, z)
+    got = fm.to_host(lat, z)                                # reference layout
+    Y, _ = nno.forward(src, nno.unpack_weights(w, lat.Ls), lat.Ls)
+    want_nat = nno.channels_to_spinor(Y, lat.Ls)            # native layout
+    want = np.moveaxis(want_nat, 2, 5)
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert err < 5e-3, err
+
+
+def test_network_gradient_matches_oracle():
+    lat, pc, w, rs = small()
+    src = spinor(rs, lat.shape).astype(np.complex64).astype(np.complex128)
+    sol = spinor(rs, lat.shape).astype(np.complex64).astype(np.complex128)
+    loss, g = pc.loss_grad(fm.spinor_dev(lat, src, "fp32"), fm.spinor_dev(lat, sol, "fp32"))
+    loss_o, g_o = nno.loss_and_grad(src, sol, nno.unpack_weights(w, lat.Ls), lat.Ls)
+    assert abs(loss - loss_o) <= 1e-3 * loss_o, (loss, loss_o)
+    err = np.linalg.norm(g - g_o) / np.linalg.norm(g_o)
+    assert err < 2e-2, err
+    # per layer as well (a layer-local indexing error would hide in the total)
+    o = 0
+    for n in (27 * 16 * 32, 27 * 16 * 16, 27 * 16 * 16, 27 * 32 * 16):
+        e = np.linalg.norm(g[o:o + n] - g_o[o:o + n]) / np.linalg.norm(g_o[o:o + n])
+        assert e < 3e-2, (o, e)
+        o += n
+
+
+def test_fcg_without_network_is_cg():
+    lat = fm.Lattice(8, 8)
+    rs = np.random.RandomState(9)
+    b, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    chid = fm.DeviceArray.from_numpy(lat, chi)
+    bd = fm.spinor_dev(lat, b)
+    x1 = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+    st_f = learn.solve_fcg(lat, bd, x1, chid, 1e-10, use_nn=False)
+    x2 = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+    st_c = fm.solve_mdagm(lat, bd, x2, chid, 1e-10, "fp32")
+    assert st_f["true_resid"] <= 1e-10 and st_c["true_resid"] <= 1e-10
+    assert abs(st_f["iterations"] - st_c["iterations"]) <= 0.05 * st_c["iterations"] + 3, (st_f, st_c)
+
+
+def test_trained_network_gain():
+    """Training on pairs from the plain solver lowers the loss, and flexible CG
+    with the trained network reaches the FP64 true residual; the test records
+    the iteration ratio against the same solver without the network."""
+    lat = fm.Lattice(8, 8)
+    rs = np.random.RandomState(13)
+    chi = phi_field(rs, lat.kvol)
+    chid = fm.DeviceArray.from_numpy(lat, chi)
+    src, sol, _ = learn.pairs_dev(lat, chid, 16, rng_seed=3)
+    pc = learn.Preconditioner(lat, hidden=16, seed=1)
+    losses = pc.train(src, sol, 16, epochs=30, lr=2e-3)
+    assert losses[-1] < 0.7 * losses[0], losses
+    b = spinor(rs, lat.shape)
+    bd = fm.spinor_dev(lat, b)
+    x1 = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+    st_nn = learn.solve_fcg(lat, bd, x1, chid, 1e-10, use_nn=True)
+    x2 = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+    st_id = learn.solve_fcg(lat, bd, x2, chid, 1e-10, use_nn=False)
+    print("loss %.4f -> %.4f; iterations with network %d, without %d" % (losses[0], losses[-1], st_nn["iterations"],
+                                                                         st_id["iterations"]))
+    assert st_nn["true_resid"] <= 1e-10
+    assert st_nn["iterations"] < st_id["iterations"]
