@@ -78,6 +78,66 @@ def cpu_baseline(L, Ls, beta, fp64_iters, budget_s=20.0):
                       "the FP64 GPU solve" % (n, L, Ls, per_it * 1e3, fp64_iters)}
 
 
+def batch_bench(npf=4, res=1e-9, seed=11):
+    """Batched multi-shift solves (all 12 remez2g shifts x npf pseudofermion
+    fields x lattices L/Ls = 8/8, 12/16, 16/32, 24/32, FP64) in one grouped,
+    graph-replayed solve versus the same solves one at a time.  Returns a
+    dict with aggregate solves/s and algorithmic HBM GB/s of both."""
+    from chiralheisenberg_amd import fermion as fm, batch, remez
+    anum, aden = remez.load("remez2g")[:2]
+    n = len(aden)
+    rs = np.random.RandomState(seed)
+    systems, lats = [], []
+    for L, Ls in ((8, 8), (12, 16), (16, 32), (24, 32)):
+        lat = fm.Lattice(L, Ls)
+        lats.append(lat)
+        chi = fm.DeviceArray.from_numpy(lat, rs.randn(3, lat.kvol) / np.sqrt(0.475))
+        for _ in range(npf):
+            phi = fm.spinor_dev(lat, (rs.randn(*lat.shape) + 1j * rs.randn(*lat.shape)) / np.sqrt(2))
+            systems.append(batch.System(lat, phi, chi, fm.spinor_dev(lat, None), anum, aden, res))
+
+    def bytes_moved(its):
+        # per iteration: two stencils (3V each incl. q / dot), the shift update
+        # reading w, q, q_prev, x and every p_k and writing r, x and every p_k
+        tot = 0
+        for s, it in zip(systems, its):
+            V, C = s.lat.kferm * 16, 3 * s.lat.kvol * 8
+            tot += it * ((12 + 2 * n) * V + 2 * C)
+        return tot
+
+    # one at a time (the single-system path, chunked launches)
+    fm.qmrherm_dev(lats[0], systems[0].phi, res, anum, aden, systems[0].chi, systems[0].x)   # warm-up
+    lats[0].synchronize()
+    t0 = time.perf_counter()
+    its1 = []
+    for s in systems:
+        its1.append(fm.qmrherm_dev(s.lat, s.phi, res, anum, aden, s.chi, s.x))
+    for lat in lats:
+        lat.synchronize()
+    t_single = time.perf_counter() - t0
+    ref = [fm.to_host(s.lat, s.x) for s in systems]
+    # batched: the first call tunes and captures, the second is timed
+    b = batch.Batch()
+    st_first = b.qmrherm(systems)
+    t0 = time.perf_counter()
+    st = b.qmrherm(systems)
+    t_batch = time.perf_counter() - t0
+    its = [s.itercg for s in systems]
+    same = all(np.array_equal(fm.to_host(s.lat, s.x), r) for s, r in zip(systems, ref)) and its == its1
+    B = bytes_moved(its)
+    out = {"workload": "qmrherm x %d: remez2g (%d shifts) x %d pseudofermion fields x L/Ls 8/8, 12/16, 16/32, "
+                       "24/32, FP64, res %g" % (len(systems), n, npf, res),
+           "solves": len(systems), "itercg": its,
+           "one_at_a_time": {"seconds": t_single, "solves_per_s": len(systems) / t_single,
+                             "GB_per_s": B / t_single / 1e9},
+           "batched": {"seconds": t_batch, "solves_per_s": len(systems) / t_batch, "GB_per_s": B / t_batch / 1e9,
+                       "tune": st["tune"], "first_call_seconds": st_first["seconds"],
+                       "graph_replays": st["graph_replays"]},
+           "speedup": t_single / t_batch, "identical_results": bool(same)}
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,12 +151,16 @@ def main():
     ap.add_argument("--delta", type=float, default=0.0, help="reliable-update threshold (0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-dslash", action="store_true", help="only the roofline kernel and a calibration copy")
+    ap.add_argument("--batch", action="store_true", help="only the batched multi-shift workload (one JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box of this deployment has one MI355X, and the path "
               "shards by independent lattice/chain, so --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.batch:
+        print(json.dumps(batch_bench()), flush=True)
+        return
     from chiralheisenberg_amd import fermion as fm
     lat = fm.Lattice(args.L, args.Ls)
     if args.only_dslash:

@@ -44,7 +44,7 @@ class MsSystem(ctypes.Structure):
 
 
 class BatchTune(ctypes.Structure):
-    _fields_ = [("split", ctypes.c_int), ("chunk", ctypes.c_int), ("graph", ctypes.c_int),
+    _fields_ = [("ept", ctypes.c_int), ("chunk", ctypes.c_int), ("graph", ctypes.c_int),
                 ("us_per_iter", ctypes.c_double)]
 
 

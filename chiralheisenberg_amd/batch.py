@@ -47,7 +47,7 @@ class Batch:
             a.aden = s.aden.ctypes.data_as(ctypes.c_void_p)
         t = None
         if tune is not None:
-            t = _lib.BatchTune(int(tune["split"]), int(tune["chunk"]), int(tune["graph"]), 0.0)
+            t = _lib.BatchTune(int(tune["ept"]), int(tune["chunk"]), int(tune["graph"]), 0.0)
         st = _lib.BatchStats()
         self._check(lib().chs_qmrherm_batch_dev(self.h, arr, len(systems), ctypes.byref(t) if t else None,
                                                  ctypes.byref(st)))
@@ -56,7 +56,7 @@ class Batch:
         tu = st.tune
         return {"iterations": st.iterations, "graph_replays": st.graph_replays, "launches": st.launches,
                 "seconds": st.seconds,
-                "tune": {"split": tu.split, "chunk": tu.chunk, "graph": tu.graph, "us_per_iter": tu.us_per_iter}}
+                "tune": {"ept": tu.ept, "chunk": tu.chunk, "graph": tu.graph, "us_per_iter": tu.us_per_iter}}
 
     def close(self):
         if self.h:

@@ -386,10 +386,12 @@ int chs_qmrherm_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
     int rc = ensure_spinors(c, {B_R, B_Q0, B_Q1, B_V, B_W});
     if (rc) return rc;
     if ((rc = ensure(c, c->shifts_p, (size_t)ndiag * n * 16))) return rc;
-    if ((rc = ensure(c, c->shifts_x, (size_t)ndiag * n * 16))) return rc;
+    // iflag = 0 accumulates the rational combination directly into x; only
+    // the force (iflag = 1) needs every x_k
+    if (iflag == 1 && (rc = ensure(c, c->shifts_x, (size_t)ndiag * n * 16))) return rc;
     double2 *r = buf<double2>(c, B_R), *q0 = buf<double2>(c, B_Q0), *q1 = buf<double2>(c, B_Q1);
     double2 *v = buf<double2>(c, B_V), *w = buf<double2>(c, B_W);
-    double2 *pk = (double2 *)c->shifts_p.p, *xk = (double2 *)c->shifts_x.p;
+    double2 *pk = (double2 *)c->shifts_p.p, *xk = iflag == 1 ? (double2 *)c->shifts_x.p : (double2 *)x;
 
     // scalar state: aden, ndiag; coefficients anum for the combine / force
     LzState h;
@@ -423,18 +425,21 @@ int chs_qmrherm_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
                                                   epi(c->partial, nullptr, qc, nullptr, nullptr, skip));
             hipLaunchKernelGGL(k_lz_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->lz, (const double *)c->partial,
                                np1, iter);
-            if (ndiag == 25)
-                hipLaunchKernelGGL(k_lz_update<25>, dim3(np2), dim3(kBlock), 0, c->stream, n, (const double2 *)w,
-                                   (const double2 *)qc, (const double2 *)qp, r, pk, xk, (const LzState *)c->lz,
-                                   ndiag, iter, c->partial2);
-            else if (ndiag == 12)
-                hipLaunchKernelGGL(k_lz_update<12>, dim3(np2), dim3(kBlock), 0, c->stream, n, (const double2 *)w,
-                                   (const double2 *)qc, (const double2 *)qp, r, pk, xk, (const LzState *)c->lz,
-                                   ndiag, iter, c->partial2);
-            else
-                hipLaunchKernelGGL(k_lz_update<0>, dim3(np2), dim3(kBlock), 0, c->stream, n, (const double2 *)w,
-                                   (const double2 *)qc, (const double2 *)qp, r, pk, xk, (const LzState *)c->lz,
-                                   ndiag, iter, c->partial2);
+            const double *an = (const double *)c->dcoef;
+            const LzState *lzc = c->lz;
+#define LZ_UPD(NS, CMB)                                                                                        \
+    hipLaunchKernelGGL((k_lz_update<NS, CMB>), dim3(np2), dim3(kBlock), 0, c->stream, n, (const double2 *)w,   \
+                       (const double2 *)qc, (const double2 *)qp, r, pk, xk, an, lzc, ndiag, iter, c->partial2)
+            if (iflag == 0) {
+                if (ndiag == 25) LZ_UPD(25, true);
+                else if (ndiag == 12) LZ_UPD(12, true);
+                else LZ_UPD(0, true);
+            } else {
+                if (ndiag == 25) LZ_UPD(25, false);
+                else if (ndiag == 12) LZ_UPD(12, false);
+                else LZ_UPD(0, false);
+            }
+#undef LZ_UPD
             hipLaunchKernelGGL(k_lz_beta, dim3(1), dim3(kBlock), 0, c->stream, c->lz, (const double *)c->partial2,
                                np2, iter);
         }
@@ -445,10 +450,7 @@ int chs_qmrherm_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
     if ((rc = poll_int(c, &c->lz->itercg, &it))) return rc;
     if (itercg) *itercg = it;
 
-    if (iflag == 0) {
-        hipLaunchKernelGGL(k_lz_combine, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, (const double2 *)phi,
-                           (const double2 *)xk, ndiag, (const double *)c->dcoef, (double2 *)x);
-    } else {
+    if (iflag == 1) {
         const size_t cb = (size_t)ndiag * 3 * c->kvol * sizeof(double);
         if ((rc = ensure(c, c->contrib, cb))) return rc;
         hipLaunchKernelGGL(k_force_shift, dim3(nblocks((long)c->kvol * ndiag)), dim3(kBlock), 0, c->stream, c->lat,

@@ -12,10 +12,9 @@
 // partial sum lands in the same slot with the same value and a batched solve
 // reproduces the one-at-a-time solve bit for bit.
 //
-// The shift update may be split over G workgroups per block of elements
-// (shift k handled by group k mod G, the residual and its norm by group 0):
-// more workgroups for small lattices, no change in any value.  The
-// iteration counter lives on the device (itercg), and the Lanczos vector
+// The shift update may cover EPT element blocks per workgroup (fewer, fatter
+// workgroups; one partial per element block, so no change in any value).
+// The iteration counter lives on the device (itercg), and the Lanczos vector
 // pair alternates with the parity of the iteration, passed as an argument,
 // so a captured sequence of an even number of iterations can be replayed.
 #pragma once
@@ -30,7 +29,9 @@ struct MsSys {
     double hpin;
     int imass, ndiag;
     int np1, np2;                 // partial counts: stencil grid, vector grid
-    double2 *r, *q0, *q1, *v, *w, *pk, *xk;
+    double2 *r, *q0, *q1, *v, *w, *pk;
+    double2 *x;                   // the caller's output: a0 phi + sum_k anum[k+1] x_k
+    const double *anum;           // device copy of the remez numerators
     LzState *lz;
     double *partial, *partial2;
 };
@@ -38,8 +39,8 @@ struct MsSys {
 struct MsGroup {
     const MsSys *sys;
     const unsigned *s_dsl;        // first block of each system in the stencil grid
-    const unsigned *s_vec;        // ... in the shift-update grid (G blocks per element block)
-    int nsys, G;
+    const unsigned *s_vec;        // ... in the shift-update grid (np2 / EPT blocks each)
+    int nsys;
     int *status;                  // [0] systems done, [1] largest itercg
 };
 
@@ -87,75 +88,28 @@ __global__ void k_ms_beta(MsGroup gr) {
     lz_beta(s.lz, s.partial2, s.np2, s.lz->itercg + 1);
 }
 
-// Shift update of one element block for shift group g of G: the same values
-// as lz_update (group 0 also forms r and the |r|^2 partial).
-template <int NS>
-__device__ __forceinline__ void lz_update_split(const MsSys &s, const double2 *__restrict__ q,
-                                                const double2 *__restrict__ qm1, int iter, long lb, int g, int G) {
-    const LzState *st = s.lz;
-    const int ndiag = s.ndiag;
-    const long n = s.n;
-    __shared__ double s_amu[kMaxShift], s_rho[kMaxShift];
-    if (threadIdx.x < ndiag) {
-        s_amu[threadIdx.x] = st->amu[threadIdx.x];
-        s_rho[threadIdx.x] = st->rho[threadIdx.x];
-    }
-    __syncthreads();
-    const long i = lb * kBlock + threadIdx.x;
-    double part = 0.0;
-    if (i < n) {
-        const double2 qv = q[i];
-        if (g == 0) {
-            const double at = st->alphatild, b0 = st->beta;
-            const double2 wv = s.w[i];
-            const double2 qm = qm1[i];
-            double2 rv;
-            rv.x = wv.x - at * qv.x - b0 * qm.x;
-            rv.y = wv.y - at * qv.y - b0 * qm.y;
-            s.r[i] = rv;
-            part = rv.x * rv.x + rv.y * rv.y;
-        }
-        const int ns = NS > 0 ? NS : ndiag;
-        double2 *__restrict__ p = s.pk;
-        double2 *__restrict__ x = s.xk;
-        if (iter == 1) {
-            for (int k = g; k < ns; k += G) {
-                p[(long)k * n + i] = qv;
-                x[(long)k * n + i] = make_double2(s_rho[k] * qv.x, s_rho[k] * qv.y);
-            }
-        } else {
-            for (int k = g; k < ns; k += G) {
-                double2 pv = p[(long)k * n + i];
-                double2 xv = x[(long)k * n + i];
-                pv.x = qv.x - s_amu[k] * pv.x;
-                pv.y = qv.y - s_amu[k] * pv.y;
-                xv.x += s_rho[k] * pv.x;
-                xv.y += s_rho[k] * pv.y;
-                p[(long)k * n + i] = pv;
-                x[(long)k * n + i] = xv;
-            }
-        }
-    }
-    if (g == 0) {
-        const double t = block_sum(part);
-        if (threadIdx.x == 0) s.partial2[lb] = t;
-    }
+// Shift update with EPT element blocks per workgroup (the partials do not
+// depend on EPT); the rational combination accumulates directly into x.
+template <int EPT>
+__device__ __forceinline__ void ms_update(const MsSys &s, int parity, long lb) {
+    const double2 *qc = parity ? s.q0 : s.q1;
+    const double2 *qp = parity ? s.q1 : s.q0;
+    const int iter = s.lz->itercg + 1;
+    if (s.ndiag == 25)
+        lz_update<25, true, EPT>(s.n, s.w, qc, qp, s.r, s.pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
+    else if (s.ndiag == 12)
+        lz_update<12, true, EPT>(s.n, s.w, qc, qp, s.r, s.pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
+    else
+        lz_update<0, true, EPT>(s.n, s.w, qc, qp, s.r, s.pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
 }
 
+template <int EPT>
 __global__ __launch_bounds__(kBlock) void k_ms_update(MsGroup gr, int parity) {
     const unsigned b = logical_block_x();
     const int j = find_sys(gr.s_vec, gr.nsys, b);
     const MsSys &s = gr.sys[j];
     if (s.lz->done) return;
-    const unsigned local = b - gr.s_vec[j];
-    const int g = (int)(local % (unsigned)gr.G);
-    const long lb = (long)(local / (unsigned)gr.G);
-    const double2 *qc = parity ? s.q0 : s.q1;
-    const double2 *qp = parity ? s.q1 : s.q0;
-    const int iter = s.lz->itercg + 1;
-    if (s.ndiag == 25) lz_update_split<25>(s, qc, qp, iter, lb, g, gr.G);
-    else if (s.ndiag == 12) lz_update_split<12>(s, qc, qp, iter, lb, g, gr.G);
-    else lz_update_split<0>(s, qc, qp, iter, lb, g, gr.G);
+    ms_update<EPT>(s, parity, (long)(b - gr.s_vec[j]));
 }
 
 // status[0] = systems done, status[1] = largest itercg (one workgroup)

@@ -281,66 +281,119 @@ __global__ void k_lz_beta(LzState *st, const double *__restrict__ partial, int n
     lz_beta(st, partial, np, iter);
 }
 
-// r = w - alphatild q - beta0 qm1 (|r|^2 partial) and, for every shift,
-// p_k = q - mu_k p_k, x_k += rho_k p_k (iter 1: p_k = q, x_k = rho_k q).
-// p and x are shift-major: shift k at offset k*n.
-template <int NS>
-__device__ __forceinline__ void lz_update(long n, const double2 *__restrict__ w, const double2 *__restrict__ q,
-                                          const double2 *__restrict__ qm1, double2 *__restrict__ r,
-                                          double2 *__restrict__ p, double2 *__restrict__ x,
-                                          const LzState *__restrict__ st, int ndiag, int iter,
-                                          double *__restrict__ partial, long lb) {
-    __shared__ double s_amu[kMaxShift], s_rho[kMaxShift];
+// Shift update of one element i: r = w - alphatild q - beta0 qm1 and, for
+// every shift, p_k = q - mu_k p_k (iter 1: p_k = q) and
+//   CMB = false: x_k += rho_k p_k (iter 1: x_k = rho_k q), x shift-major
+//                (shift k at offset k*n);
+//   CMB = true:  x += sum_k c_k p_k with c_k = anum[k+1] rho_k, x the single
+//                running rational combination a0 phi + sum_k anum[k+1] x_k,
+//                which is all qmrherm's iflag = 0 returns: the x_k are never
+//                stored, which halves the shift traffic.
+// Coefficients come from shared memory (lz_coefs).  Returns |r_i|^2.
+struct LzCoef {
+    double amu[kMaxShift], rho[kMaxShift], c[kMaxShift];
+};
+
+template <bool CMB>
+__device__ __forceinline__ void lz_coefs(LzCoef &sc, const LzState *__restrict__ st, int ndiag,
+                                         const double *__restrict__ anum) {
     if (threadIdx.x < ndiag) {
-        s_amu[threadIdx.x] = st->amu[threadIdx.x];
-        s_rho[threadIdx.x] = st->rho[threadIdx.x];
+        sc.amu[threadIdx.x] = st->amu[threadIdx.x];
+        sc.rho[threadIdx.x] = st->rho[threadIdx.x];
+        if (CMB) sc.c[threadIdx.x] = anum[threadIdx.x + 1] * st->rho[threadIdx.x];
     }
     __syncthreads();
-    const long i = lb * kBlock + threadIdx.x;
-    double part = 0.0;
-    if (i < n) {
-        const double at = st->alphatild, b0 = st->beta;
-        const double2 qv = q[i], wv = w[i];
-        const double2 qm = qm1[i];
-        double2 rv;
-        rv.x = wv.x - at * qv.x - b0 * qm.x;
-        rv.y = wv.y - at * qv.y - b0 * qm.y;
-        r[i] = rv;
-        part = rv.x * rv.x + rv.y * rv.y;
-        const int ns = NS > 0 ? NS : ndiag;
+}
+
+template <int NS, bool CMB>
+__device__ __forceinline__ double lz_update_elem(long i, long n, const double2 *__restrict__ w,
+                                                 const double2 *__restrict__ q, const double2 *__restrict__ qm1,
+                                                 double2 *__restrict__ r, double2 *__restrict__ p,
+                                                 double2 *__restrict__ x, const LzState *__restrict__ st,
+                                                 const LzCoef &sc, int ndiag, int iter) {
+    const double at = st->alphatild, b0 = st->beta;
+    const double2 qv = q[i], wv = w[i];
+    const double2 qm = qm1[i];
+    double2 rv;
+    rv.x = wv.x - at * qv.x - b0 * qm.x;
+    rv.y = wv.y - at * qv.y - b0 * qm.y;
+    r[i] = rv;
+    const int ns = NS > 0 ? NS : ndiag;
+    if (CMB) {
+        double2 xv = x[i];
         if (iter == 1) {
 #pragma unroll 4
             for (int k = 0; k < ns; k++) {
                 p[(long)k * n + i] = qv;
-                x[(long)k * n + i] = make_double2(s_rho[k] * qv.x, s_rho[k] * qv.y);
+                xv.x += sc.c[k] * qv.x;
+                xv.y += sc.c[k] * qv.y;
             }
         } else {
 #pragma unroll 4
             for (int k = 0; k < ns; k++) {
                 double2 pv = p[(long)k * n + i];
-                double2 xv = x[(long)k * n + i];
-                pv.x = qv.x - s_amu[k] * pv.x;
-                pv.y = qv.y - s_amu[k] * pv.y;
-                xv.x += s_rho[k] * pv.x;
-                xv.y += s_rho[k] * pv.y;
+                pv.x = qv.x - sc.amu[k] * pv.x;
+                pv.y = qv.y - sc.amu[k] * pv.y;
                 p[(long)k * n + i] = pv;
-                x[(long)k * n + i] = xv;
+                xv.x += sc.c[k] * pv.x;
+                xv.y += sc.c[k] * pv.y;
             }
         }
+        x[i] = xv;
+    } else if (iter == 1) {
+#pragma unroll 4
+        for (int k = 0; k < ns; k++) {
+            p[(long)k * n + i] = qv;
+            x[(long)k * n + i] = make_double2(sc.rho[k] * qv.x, sc.rho[k] * qv.y);
+        }
+    } else {
+#pragma unroll 4
+        for (int k = 0; k < ns; k++) {
+            double2 pv = p[(long)k * n + i];
+            double2 xv = x[(long)k * n + i];
+            pv.x = qv.x - sc.amu[k] * pv.x;
+            pv.y = qv.y - sc.amu[k] * pv.y;
+            xv.x += sc.rho[k] * pv.x;
+            xv.y += sc.rho[k] * pv.y;
+            p[(long)k * n + i] = pv;
+            x[(long)k * n + i] = xv;
+        }
     }
-    const double t = block_sum(part);
-    if (threadIdx.x == 0) partial[lb] = t;
+    return rv.x * rv.x + rv.y * rv.y;
 }
 
-template <int NS>
+// EPT consecutive element blocks of kBlock per workgroup (block lb covers
+// element blocks lb*EPT .. lb*EPT+EPT-1), one |r|^2 partial per element block,
+// so the partials do not depend on EPT.
+template <int NS, bool CMB, int EPT>
+__device__ __forceinline__ void lz_update(long n, const double2 *__restrict__ w, const double2 *__restrict__ q,
+                                          const double2 *__restrict__ qm1, double2 *__restrict__ r,
+                                          double2 *__restrict__ p, double2 *__restrict__ x,
+                                          const double *__restrict__ anum, const LzState *__restrict__ st, int ndiag,
+                                          int iter, double *__restrict__ partial, long lb) {
+    __shared__ LzCoef sc;
+    lz_coefs<CMB>(sc, st, ndiag, anum);
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const long eb = lb * EPT + e;
+        const long i = eb * kBlock + threadIdx.x;
+        double part = 0.0;
+        if (i < n) part = lz_update_elem<NS, CMB>(i, n, w, q, qm1, r, p, x, st, sc, ndiag, iter);
+        const double t = block_sum(part);
+        if (threadIdx.x == 0) partial[eb] = t;
+    }
+}
+
+template <int NS, bool CMB>
 __global__ __launch_bounds__(kBlock) void k_lz_update(long n, const double2 *__restrict__ w,
                                                       const double2 *__restrict__ q,
                                                       const double2 *__restrict__ qm1, double2 *__restrict__ r,
                                                       double2 *__restrict__ p, double2 *__restrict__ x,
+                                                      const double *__restrict__ anum,
                                                       const LzState *__restrict__ st, int ndiag, int iter,
                                                       double *__restrict__ partial) {
     if (st->done) return;
-    lz_update<NS>(n, w, q, qm1, r, p, x, st, ndiag, iter, partial, logical_block());
+    lz_update<NS, CMB, 1>(n, w, q, qm1, r, p, x, anum, st, ndiag, iter, partial, logical_block());
 }
 
 // x = a0 phi + sum_k a_k x_k, summed in shift order (rhmc/cheisdwf.f:722-727)

@@ -200,7 +200,10 @@ int chs_nn_train_dev(chs_ctx *ctx, const float *src, const float *sol, int npair
                      double *epoch_loss);
 /* Flexible CG (FP32 iteration, FP64 reliable updates every delta of
  * residual reduction, answer judged by the FP64 true relative residual) with
- * z = N(r) (use_nn = 1) or z = r (use_nn = 0, plain CG). */
+ * z = N(r) (use_nn = 1) or z = r (use_nn = 0, plain CG).  Plain-CG fallback:
+ * when with the network |r|^2 has not fallen by a factor 8 within 64
+ * iterations the solve continues from its current x as plain CG
+ * (stats->restarts counts it). */
 int chs_solve_mdagm_fcg_dev(chs_ctx *ctx, const double *b, double *x, const double *chi, double hpin, int imass,
                             double tol, double delta, int maxiter, int use_nn, chs_solve_stats *stats);
 
@@ -225,11 +228,11 @@ typedef struct {
     double res;             /* qmrherm's res: stop when max_k |rho_k| phimod < 2 sqrt(kferm) res */
     int itercg;             /* out */
 } chs_ms_system;
-/* launch parameters: split = workgroups sharing one element block of the
- * shift update (1..8), chunk = iterations per poll (even), graph = replay
- * chunks from a hipGraph; us_per_iter = the tuner's measurement */
+/* launch parameters: ept = element blocks (of 256) per workgroup of the
+ * shift update (1, 2 or 4), chunk = iterations per poll (even), graph =
+ * replay chunks from a hipGraph; us_per_iter = the tuner's measurement */
 typedef struct {
-    int split, chunk, graph;
+    int ept, chunk, graph;
     double us_per_iter;
 } chs_batch_tune;
 typedef struct {

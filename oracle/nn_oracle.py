@@ -117,17 +117,11 @@ def forward(src_ref, weights, Ls):
     return Y, (W, X0, A1, A2, A3)
 
 
-def loss_and_grad(src_ref, sol_ref, weights, Ls):
-    """MSE loss against sol / rms(sol) and the gradients of all weights
-    (concatenated like the library's weight vector)."""
-    Y, (W, X0, A1, A2, A3) = forward(src_ref, weights, Ls)
-    natx = native_from_reference(sol_ref.astype(np.complex64 if ROUND else np.complex128))
-    tsc = np.float32(1.0 / rms(natx.astype(np.complex128))) if ROUND else 1.0 / rms(natx)
-    T = spinor_to_channels(natx).astype(np.float64) * tsc
-    nreal = T.size
-    diff = (Y.astype(np.float32).astype(np.float64) if ROUND else Y) - T
-    loss = np.sum(diff * diff) / nreal
-    G4 = (2.0 / nreal * diff).astype(np.float32 if ROUND else np.float64)
+def backward(cache, G4):
+    """Gradients of all weights (concatenated like the library's weight
+    vector) from the gradient G4 (Lt,L,L,Cin) of a loss with respect to the
+    network output channels."""
+    W, X0, A1, A2, A3 = cache
     dW4 = conv_weight_grad(A3, bf16(G4))
     dA3 = bf16(conv_data_grad(bf16(G4), W[3]))
     dZ3 = bf16(dA3 * (1 - A3 * A3))
@@ -138,5 +132,18 @@ def loss_and_grad(src_ref, sol_ref, weights, Ls):
     dA1 = bf16(conv_data_grad(dZ2, W[1]))
     dZ1 = bf16(dA1 * (1 - A1 * A1))
     dW1 = conv_weight_grad(X0, dZ1)
-    grad = np.concatenate([g.reshape(-1) for g in (dW1, dW2, dW3, dW4)])
-    return loss, grad
+    return np.concatenate([g.reshape(-1) for g in (dW1, dW2, dW3, dW4)])
+
+
+def loss_and_grad(src_ref, sol_ref, weights, Ls):
+    """MSE loss against sol / rms(sol) and the gradients of all weights
+    (concatenated like the library's weight vector)."""
+    Y, cache = forward(src_ref, weights, Ls)
+    natx = native_from_reference(sol_ref.astype(np.complex64 if ROUND else np.complex128))
+    tsc = np.float32(1.0 / rms(natx.astype(np.complex128))) if ROUND else 1.0 / rms(natx)
+    T = spinor_to_channels(natx).astype(np.float64) * tsc
+    nreal = T.size
+    diff = (Y.astype(np.float32).astype(np.float64) if ROUND else Y) - T
+    loss = np.sum(diff * diff) / nreal
+    G4 = (2.0 / nreal * diff).astype(np.float32 if ROUND else np.float64)
+    return loss, backward(cache, G4)

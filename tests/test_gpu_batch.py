@@ -37,8 +37,8 @@ def single(s):
     return fm.to_host(s.lat, x), it
 
 
-@pytest.mark.parametrize("tune", [None, {"split": 1, "chunk": 2, "graph": 0}, {"split": 4, "chunk": 6, "graph": 1},
-                                  {"split": 2, "chunk": 16, "graph": 1}])
+@pytest.mark.parametrize("tune", [None, {"ept": 1, "chunk": 2, "graph": 0}, {"ept": 4, "chunk": 6, "graph": 1},
+                                  {"ept": 2, "chunk": 16, "graph": 1}])
 def test_batch_equals_single(tune):
     lats, systems = make_systems()
     b = batch.Batch()
