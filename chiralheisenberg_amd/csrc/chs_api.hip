@@ -16,8 +16,10 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace chs;
@@ -42,6 +44,8 @@ struct chs_ctx {
     void *hpinned = nullptr;      // pinned host scratch for polling
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     struct NNState *nn = nullptr; // learned preconditioner (chs_learn_host.inc)
+    int use_graphs = 1;           // replay solver iteration chunks from hipGraphs (CHS_GRAPHS=0: off)
+    int pair_stencil = 1;         // paired-site FP32 stencil on even L (CHS_PAIRS=0: off)
 };
 static void nn_destroy(chs_ctx *c);
 
@@ -73,6 +77,14 @@ inline unsigned nblocks(long n) {
     nb = (nb + 7) / 8 * 8;   // multiple of the XCD count for logical_block()
     return (unsigned)(nb > 0 ? nb : 8);
 }
+
+// owns an instantiated graph for the duration of a solve
+struct GraphGuard {
+    hipGraphExec_t exec = nullptr;
+    ~GraphGuard() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
 
 int ensure(chs_ctx *c, chs_ctx::Buf &b, size_t bytes) {
     if (b.n >= bytes) return CHS_OK;
@@ -130,9 +142,31 @@ int d2h_spinor(chs_ctx *c, void *host, const void *src) {
 size_t prec_bytes(int prec) { return prec == CHS_FP64 ? 16 : prec == CHS_FP32 ? 8 : 4; }
 
 // ---- launchers --------------------------------------------------------------
+// FP32 storage on an even L uses the paired-site stencil (measured at L=24,
+// Ls=32: FP32 17.5 -> 13.1 us; BF16 pairs, 8-byte loads at lower occupancy,
+// were slower than its one-site stencil: 10.2 -> 11.3 us)
+template <class S>
+bool use_pairs(const chs_ctx *c) {
+    return std::is_same<S, F32>::value && c->pair_stencil && (c->L % 2 == 0);
+}
+
+// workgroups (= partials) of the stencil launch dslash_launch<S> makes
+template <class S>
+unsigned dsl_grid(const chs_ctx *c) {
+    return use_pairs<S>(c) ? stencil_grid_pair(c->n5) : stencil_grid(c->n5);
+}
+
 template <class S, class SA, bool DAG, int EPI>
 void dslash_launch(chs_ctx *c, void *out, const void *in, const double *chi, double hpin, int imass,
                    EpiArgs ea) {
+    if constexpr (std::is_same<S, F32>::value) {
+        if (use_pairs<S>(c)) {
+            hipLaunchKernelGGL((k_dsl_pair<S, SA, DAG, EPI>), dim3(stencil_grid_pair(c->n5)), dim3(kBlock), 0,
+                               c->stream, c->lat, (typename S::st *)out, (const typename S::st *)in, chi, hpin,
+                               imass, ea);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_dsl<S, SA, DAG, EPI>), dim3(stencil_grid(c->n5)), dim3(kBlock), 0, c->stream, c->lat,
                        (typename S::st *)out, (const typename S::st *)in, chi, hpin, imass, ea);
 }
@@ -231,6 +265,8 @@ chs_ctx *chs_ctx_create(int L, int Lt, int Ls, int ibound, double am3, int devic
     c->lat.diag = 4.0 - am3;
     c->lat.apbc = (ibound == -1);
     c->lat.ps = c->n5;
+    if (const char *g = getenv("CHS_GRAPHS")) c->use_graphs = atoi(g) != 0;
+    if (const char *g = getenv("CHS_PAIRS")) c->pair_stencil = atoi(g) != 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->lz, sizeof(LzState)) != hipSuccess || hipMalloc(&c->cg, sizeof(CgState)) != hipSuccess ||
         hipMalloc(&c->dscal, 64 * sizeof(double)) != hipSuccess ||
@@ -667,7 +703,7 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
     const double2 *b = (const double2 *)bvec;
     double2 *x = (double2 *)xvec;
     const unsigned gn = nblocks(n);
-    const int np = (int)stencil_grid(c->n5);
+    const int np = (int)dsl_grid<S>(c);
 
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     reduce_launch(c, n, b, nullptr, 0, c->dscal);
@@ -706,7 +742,9 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
         HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
         const int *skip = &c->cg->halt;
         const int chunk = chunk_for(c, 11.0 * bs * n);
-        for (;;) {
+        // one chunk of iterations: five launches each, replayed from a
+        // hipGraph (the pointers are fixed for the whole solve)
+        auto enqueue_chunk = [&]() {
             for (int k = 0; k < chunk; k++) {
                 dslash_launch<S, SA, false, EPI_NORM>(c, mp, ps, chi, hpin, imass,
                                                      epi(c->partial, nullptr, nullptr, nullptr, nullptr, skip));
@@ -719,6 +757,20 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
                 hipLaunchKernelGGL((k_cg_pupdate<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
                                    (typename S::st *)ps, (const typename SA::st *)rs, (const CgState *)c->cg);
             }
+        };
+        GraphGuard gg;
+        if (c->use_graphs) {
+            hipGraph_t graph;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_chunk();
+            HIPCHK(c, hipStreamEndCapture(c->stream, &graph));
+            const hipError_t e = hipGraphInstantiate(&gg.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPCHK(c, e);
+        }
+        for (;;) {
+            if (gg.exec) HIPCHK(c, hipGraphLaunch(gg.exec, c->stream));
+            else enqueue_chunk();
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipMemcpyAsync(&h, c->cg, sizeof h, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -59,6 +59,14 @@ struct F32 {
         return {v.x, v.y};
     }
     static __device__ __forceinline__ void sto(st *p, long i, cx<float> v) { p[i] = make_float2(v.r, v.i); }
+    // elements i, i+1 (i even) in one 16-byte access
+    typedef float4 st2;
+    static __device__ __forceinline__ void ld2(const st *p, long i, cx<float> &a, cx<float> &b) {
+        const float4 v = *reinterpret_cast<const float4 *>(p + i);
+        a = {v.x, v.y};
+        b = {v.z, v.w};
+    }
+    static __device__ __forceinline__ st2 pack2(cx<float> a, cx<float> b) { return make_float4(a.r, a.i, b.r, b.i); }
 };
 __device__ __forceinline__ uint32_t f2bf(float x) {
     // round to nearest even; inputs are finite
@@ -74,6 +82,16 @@ struct B16 {
         return {__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u)};
     }
     static __device__ __forceinline__ void sto(st *p, long i, cx<float> v) { p[i] = f2bf(v.r) | (f2bf(v.i) << 16); }
+    // elements i, i+1 (i even) in one 8-byte access
+    typedef uint2 st2;
+    static __device__ __forceinline__ void ld2(const st *p, long i, cx<float> &a, cx<float> &b) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(p + i);
+        a = {__uint_as_float(v.x << 16), __uint_as_float(v.x & 0xffff0000u)};
+        b = {__uint_as_float(v.y << 16), __uint_as_float(v.y & 0xffff0000u)};
+    }
+    static __device__ __forceinline__ st2 pack2(cx<float> a, cx<float> b) {
+        return make_uint2(f2bf(a.r) | (f2bf(a.i) << 16), f2bf(b.r) | (f2bf(b.i) << 16));
+    }
 };
 
 // Blocks are handed to the 8 XCDs round-robin; give each XCD a contiguous
@@ -141,10 +159,20 @@ struct EpiArgs {
 };
 
 // ---- deterministic FP64 reductions -----------------------------------------
-// One workgroup sums n partials in a fixed order.
+// One workgroup sums n partials in a fixed order: thread t adds part[t],
+// part[t + kBlock], ... in that order, then block_sum.  The loads are issued
+// eight at a time ahead of the additions (the sum is latency-bound).
 __device__ __forceinline__ double sum_partials(const double *__restrict__ part, int n) {
     double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += kBlock) v += part[i];
+    int i = threadIdx.x;
+    for (; i + 7 * kBlock < n; i += 8 * kBlock) {
+        double t[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) t[k] = part[i + k * kBlock];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v += t[k];
+    }
+    for (; i < n; i += kBlock) v += part[i];
     return block_sum(v);
 }
 
@@ -527,29 +555,33 @@ __global__ __launch_bounds__(kBlock) void k_cg_pupdate(long n, typename S::st *_
 }
 
 // reliable-update CG: alpha = rnorm2 / |M p|^2
-__global__ void k_rcg_alpha(CgState *st, const double *__restrict__ partial, int np) {
-    if (st->halt) return;
-    const double t = sum_partials(partial, np);
-    if (threadIdx.x == 0) st->alpha = st->rnorm2 / t;
-}
+__device__ __forceinline__ void rcg_alpha_step(CgState *st, double pap) { st->alpha = st->rnorm2 / pap; }
 
 // reliable-update CG after |r_s|^2 of the iterated residual: a plain step
 // (beta), or a hold for the host (reliable update due, converged candidate,
 // iteration cap, or a non-finite value).
+__device__ __forceinline__ void rcg_beta_step(CgState *st, double rn) {
+    st->iter += 1;
+    if (rn < st->delta2 * st->maxr2 || rn < st->stop2 || st->iter >= st->maxiter || !(rn == rn)) {
+        st->hold = 1;
+        st->halt = 1;
+        st->rn_hold = rn;
+    } else {
+        st->beta = rn / st->rnorm2;
+        st->rnorm2 = rn;
+    }
+}
+
+__global__ void k_rcg_alpha(CgState *st, const double *__restrict__ partial, int np) {
+    if (st->halt) return;
+    const double t = sum_partials(partial, np);
+    if (threadIdx.x == 0) rcg_alpha_step(st, t);
+}
+
 __global__ void k_rcg_beta(CgState *st, const double *__restrict__ partial, int np) {
     if (st->halt) return;
     const double rn = sum_partials(partial, np);
-    if (threadIdx.x == 0) {
-        st->iter += 1;
-        if (rn < st->delta2 * st->maxr2 || rn < st->stop2 || st->iter >= st->maxiter || !(rn == rn)) {
-            st->hold = 1;
-            st->halt = 1;
-            st->rn_hold = rn;
-        } else {
-            st->beta = rn / st->rnorm2;
-            st->rnorm2 = rn;
-        }
-    }
+    if (threadIdx.x == 0) rcg_beta_step(st, rn);
 }
 
 // x64 += xs (low precision accumulator), xs = 0

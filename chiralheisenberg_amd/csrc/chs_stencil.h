@@ -54,40 +54,56 @@ __device__ __forceinline__ void store_nt(typename S::st *__restrict__ p, unsigne
     }
 }
 
+// o += coef * (1 - e g_mu) (p0, p1, p2, p3).  MU = 0, 1, 2 (x, y, t) with
+// e = +1; MU = -1, -2, -3 the same with e = -1.
+template <class T, int MU>
+__device__ __forceinline__ void hop_vals(cx<T> (&o)[4], cx<T> p0, cx<T> p1, cx<T> p2, cx<T> p3, T coef) {
+    constexpr int mu = MU < 0 ? -MU - 1 : MU;
+    constexpr T e = MU < 0 ? T(-1) : T(1);
+    cx<T> a, b, h2, h3;
+    if (mu == 0) {          // g1: (a, b, -e i b, -e i a), a = p0 + e i p3, b = p1 + e i p2
+        a = cadd(p0, cmuli(e, p3));
+        b = cadd(p1, cmuli(e, p2));
+        h2 = cmuli(-e, b);
+        h3 = cmuli(-e, a);
+    } else if (mu == 1) {   // g2: (a, b, -e b, e a), a = p0 + e p3, b = p1 - e p2
+        a = cadd(p0, cscale(e, p3));
+        b = csub(p1, cscale(e, p2));
+        h2 = cscale(-e, b);
+        h3 = cscale(e, a);
+    } else {                // g3: (a, b, -e i a, e i b), a = p0 + e i p2, b = p1 - e i p3
+        a = cadd(p0, cmuli(e, p2));
+        b = csub(p1, cmuli(e, p3));
+        h2 = cmuli(-e, a);
+        h3 = cmuli(e, b);
+    }
+    o[0] = {o[0].r + coef * a.r, o[0].i + coef * a.i};
+    o[1] = {o[1].r + coef * b.r, o[1].i + coef * b.i};
+    o[2] = {o[2].r + coef * h2.r, o[2].i + coef * h2.i};
+    o[3] = {o[3].r + coef * h3.r, o[3].i + coef * h3.i};
+}
+
 // o[ff] += coef * (1 - e g_mu) psi_ff(gn); pl[ff][d] are uniform plane bases.
-// MU = 0, 1, 2 (x, y, t) with e = +1; MU = -1, -2, -3 the same with e = -1.
 template <class S, int MU, int NF>
 __device__ __forceinline__ void hop2(cx<typename S::rt> (&o)[NF][4], const typename S::st *const (&pl)[NF][4],
                                      unsigned gn, typename S::rt coef) {
     typedef typename S::rt T;
-    constexpr int mu = MU < 0 ? -MU - 1 : MU;
-    constexpr T e = MU < 0 ? T(-1) : T(1);
 #pragma unroll
-    for (int ff = 0; ff < NF; ff++) {
-        const cx<T> p0 = S::ld(pl[ff][0], gn), p1 = S::ld(pl[ff][1], gn);
-        const cx<T> p2 = S::ld(pl[ff][2], gn), p3 = S::ld(pl[ff][3], gn);
-        cx<T> a, b, h2, h3;
-        if (mu == 0) {          // g1: (a, b, -e i b, -e i a), a = p0 + e i p3, b = p1 + e i p2
-            a = cadd(p0, cmuli(e, p3));
-            b = cadd(p1, cmuli(e, p2));
-            h2 = cmuli(-e, b);
-            h3 = cmuli(-e, a);
-        } else if (mu == 1) {   // g2: (a, b, -e b, e a), a = p0 + e p3, b = p1 - e p2
-            a = cadd(p0, cscale(e, p3));
-            b = csub(p1, cscale(e, p2));
-            h2 = cscale(-e, b);
-            h3 = cscale(e, a);
-        } else {                // g3: (a, b, -e i a, e i b), a = p0 + e i p2, b = p1 - e i p3
-            a = cadd(p0, cmuli(e, p2));
-            b = csub(p1, cmuli(e, p3));
-            h2 = cmuli(-e, a);
-            h3 = cmuli(e, b);
-        }
-        o[ff][0] = {o[ff][0].r + coef * a.r, o[ff][0].i + coef * a.i};
-        o[ff][1] = {o[ff][1].r + coef * b.r, o[ff][1].i + coef * b.i};
-        o[ff][2] = {o[ff][2].r + coef * h2.r, o[ff][2].i + coef * h2.i};
-        o[ff][3] = {o[ff][3].r + coef * h3.r, o[ff][3].i + coef * h3.i};
-    }
+    for (int ff = 0; ff < NF; ff++)
+        hop_vals<T, MU>(o[ff], S::ld(pl[ff][0], gn), S::ld(pl[ff][1], gn), S::ld(pl[ff][2], gn),
+                        S::ld(pl[ff][3], gn), coef);
+}
+
+// the same hop for the two sites of a pair: neighbours at gn, gn + 1 (gn even)
+template <class S, int MU>
+__device__ __forceinline__ void hop_pair(cx<typename S::rt> (&o0)[4], cx<typename S::rt> (&o1)[4],
+                                         const typename S::st *const (&pl)[4], unsigned gn, typename S::rt coef) {
+    typedef typename S::rt T;
+    cx<T> a[4], b[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) S::ld2(pl[d], gn, a[d], b[d]);
+    hop_vals<T, MU>(o0, a[0], a[1], a[2], a[3], coef);
+    hop_vals<T, MU>(o1, b[0], b[1], b[2], b[3], coef);
 }
 
 // Element offset of (s, site) inside a plane.
@@ -266,6 +282,205 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     double hpin, int imass, EpiArgs ea) {
     if (ea.skip && *ea.skip) return;
     dsl_block<S, SA, DAG, EPI, LAYOUT>(lat, out, in, chi, hpin, imass, ea, logical_block_x());
+}
+
+// ---- paired-site stencil (FP32 / BF16 storage, even L) --------------------
+// One thread computes the sites x0 and x0 + 1 (x0 even) of one (s, y, t) row
+// and one flavour.  Every plane access of the centre and of the y, t and s
+// neighbours is one 16-byte (FP32) or 8-byte (BF16) load of both elements,
+// the inner x neighbours are the pair itself and only the two outer x
+// neighbours are single loads: half the load instructions per site of the
+// one-site stencil.  Each site accumulates its terms in the same order as
+// stencil(), so the outputs are identical to it.
+template <class S>
+__device__ __forceinline__ void store_nt2(typename S::st *__restrict__ p, unsigned i, typename S::st2 v) {
+    if constexpr (sizeof(v) == 16) {
+        u32x4 raw;
+        __builtin_memcpy(&raw, &v, 16);
+        __builtin_nontemporal_store(raw, reinterpret_cast<u32x4 *>(p + i));
+    } else {
+        u32x2 raw;
+        __builtin_memcpy(&raw, &v, 8);
+        __builtin_nontemporal_store(raw, reinterpret_cast<u32x2 *>(p + i));
+    }
+}
+
+template <class S, bool DAG>
+__device__ __forceinline__ void stencil_pair(const Lat &lat, const typename S::st *__restrict__ in,
+                                             const double *__restrict__ chi, double hpin, int imass, unsigned s,
+                                             unsigned site, int fb, cx<typename S::rt> (&o0)[4],
+                                             cx<typename S::rt> (&o1)[4], cx<typename S::rt> (&c0)[4],
+                                             cx<typename S::rt> (&c1)[4]) {
+    typedef typename S::rt T;
+    typedef typename S::st st;
+    const unsigned Ls = (unsigned)lat.Ls, L = (unsigned)lat.L, Lt = (unsigned)lat.Lt, kvol = (unsigned)lat.kvol;
+    const size_t P = (size_t)lat.ps;
+    const st *pl[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) pl[d] = in + P * (size_t)(d + 4 * fb);
+    const unsigned ix = site % L;
+    const unsigned r1 = site / L;
+    const unsigned iy = r1 % L;
+    const unsigned it = r1 / L;
+    const unsigned g = s * kvol + site;
+    const T dg = (T)lat.diag;
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        S::ld2(pl[d], g, c0[d], c1[d]);
+        o0[d] = cscale(dg, c0[d]);
+        o1[d] = cscale(dg, c1[d]);
+    }
+    const T mh = T(-0.5);
+    constexpr int XF = DAG ? -1 : 0, XB = DAG ? 0 : -1;
+    constexpr int YF = DAG ? -2 : 1, YB = DAG ? 1 : -2;
+    constexpr int TF = DAG ? -3 : 2, TB = DAG ? 2 : -3;
+    // x: site 0's forward and site 1's backward neighbour are the pair itself
+    {
+        const unsigned gf = (ix + 2 == L) ? g - (L - 2) : g + 2;
+        const unsigned gb = (ix == 0) ? g + (L - 1) : g - 1;
+        const cx<T> f0 = S::ld(pl[0], gf), f1 = S::ld(pl[1], gf), f2 = S::ld(pl[2], gf), f3 = S::ld(pl[3], gf);
+        const cx<T> b0 = S::ld(pl[0], gb), b1 = S::ld(pl[1], gb), b2 = S::ld(pl[2], gb), b3 = S::ld(pl[3], gb);
+        hop_vals<T, XF>(o0, c1[0], c1[1], c1[2], c1[3], mh);
+        hop_vals<T, XB>(o0, b0, b1, b2, b3, mh);
+        hop_vals<T, XF>(o1, f0, f1, f2, f3, mh);
+        hop_vals<T, XB>(o1, c0[0], c0[1], c0[2], c0[3], mh);
+    }
+    const unsigned sy = L, stt = L * L;
+    hop_pair<S, YF>(o0, o1, pl, (iy + 1 == L) ? g - (L - 1) * sy : g + sy, mh);
+    hop_pair<S, YB>(o0, o1, pl, (iy == 0) ? g + (L - 1) * sy : g - sy, mh);
+    hop_pair<S, TF>(o0, o1, pl, (it + 1 == Lt) ? g - (Lt - 1) * stt : g + stt,
+                    (lat.apbc && it + 1 == Lt) ? T(0.5) : mh);
+    hop_pair<S, TB>(o0, o1, pl, (it == 0) ? g + (Lt - 1) * stt : g - stt, (lat.apbc && it == 0) ? T(0.5) : mh);
+    constexpr int dn = DAG ? 0 : 2;
+    constexpr int dp = DAG ? 2 : 0;
+    const bool wall_ok = (imass == 1 || imass == 3);
+    const bool last = (s + 1 == Ls), first = (s == 0);
+    if (!last) {
+#pragma unroll
+        for (int dd = 0; dd < 2; dd++) {
+            cx<T> a, b;
+            S::ld2(pl[dn + dd], g + kvol, a, b);
+            o0[dn + dd] = csub(o0[dn + dd], a);
+            o1[dn + dd] = csub(o1[dn + dd], b);
+        }
+    }
+    if (!first) {
+#pragma unroll
+        for (int dd = 0; dd < 2; dd++) {
+            cx<T> a, b;
+            S::ld2(pl[dp + dd], g - kvol, a, b);
+            o0[dp + dd] = csub(o0[dp + dd], a);
+            o1[dp + dd] = csub(o1[dp + dd], b);
+        }
+    }
+    if ((first || last) && wall_ok) {
+        const double2 ch1 = *reinterpret_cast<const double2 *>(chi + site);
+        const double2 ch2 = *reinterpret_cast<const double2 *>(chi + site + kvol);
+        const double2 ch3 = *reinterpret_cast<const double2 *>(chi + site + 2u * kvol);
+        const T c1a = (T)ch1.x, c1b = (T)ch1.y, c2a = (T)ch2.x, c2b = (T)ch2.y;
+        double c3ad = ch3.x;
+        if (imass == 1 && site == 0) c3ad += hpin;
+        const T c3a = (T)c3ad, c3b = (T)ch3.y;
+        const st *q0 = in, *q1 = in + 4 * P;   // flavour 0 / 1 planes at d = 0
+        const unsigned src = last ? site : (Ls - 1) * kvol + site;
+#pragma unroll
+        for (int dd = 0; dd < 2; dd++) {
+            const int d = last ? dn + dd : dp + dd;
+            cx<T> v0a, v0b, v1a, v1b;
+            S::ld2(q0 + P * (size_t)d, src, v0a, v0b);
+            S::ld2(q1 + P * (size_t)d, src, v1a, v1b);
+            cx<T> w0a = {0, 0}, w1a = {0, 0}, w0b = {0, 0}, w1b = {0, 0};
+            wall_apply<T>(w0a, w1a, v0a, v1a, c1a, c2a, c3a, imass, last);
+            wall_apply<T>(w0b, w1b, v0b, v1b, c1b, c2b, c3b, imass, last);
+            o0[d] = cadd(o0[d], fb ? w1a : w0a);
+            o1[d] = cadd(o1[d], fb ? w1b : w0b);
+        }
+    }
+}
+
+__host__ __device__ inline unsigned stencil_pair_blocks_per_flavour(long n5) {
+    return (unsigned)((n5 / 2 + kBlock - 1) / kBlock);
+}
+
+template <class S, class SA, bool DAG, int EPI>
+__device__ __forceinline__ void dsl_block_pair(const Lat &lat, typename S::st *__restrict__ out,
+                                               const typename S::st *__restrict__ in,
+                                               const double *__restrict__ chi, double hpin, int imass,
+                                               const EpiArgs &ea, unsigned lb) {
+    typedef typename S::rt T;
+    typedef typename SA::rt TA;
+    const unsigned bpf = stencil_pair_blocks_per_flavour(lat.n5);
+    const int fb = (int)(lb / bpf);
+    const unsigned g2 = (lb - (unsigned)fb * bpf) * kBlock + threadIdx.x;   // pair index in (s, site) order
+    const size_t P = (size_t)lat.ps;
+    double part = 0.0;
+    if (fb < 2 && g2 < (unsigned)(lat.n5 / 2)) {
+        const unsigned kvol = (unsigned)lat.kvol;
+        const unsigned g = 2 * g2;
+        const unsigned s = g / kvol;
+        const unsigned site = g - s * kvol;
+        cx<T> o0[4], o1[4], c0[4], c1[4];
+        stencil_pair<S, DAG>(lat, in, chi, hpin, imass, s, site, fb, o0, o1, c0, c1);
+        T acc = 0;
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const size_t po = P * (size_t)(d + 4 * fb);
+            if (EPI == EPI_SCALEQ) {
+                const T ib = (T)(1.0 / ea.scal[0]);
+                store_nt2<S>(out + po, g, S::pack2(cscale(ib, o0[d]), cscale(ib, o1[d])));
+                store_nt2<S>((typename S::st *)ea.aux0 + po, g, S::pack2(cscale(ib, c0[d]), cscale(ib, c1[d])));
+            } else if (EPI == EPI_CG || EPI == EPI_CGQ) {
+                if (EPI == EPI_CGQ) store_nt2<S>(out + po, g, S::pack2(o0[d], o1[d]));
+                const TA alpha = (TA)ea.scal[0];
+                typename SA::st *xr = (typename SA::st *)ea.aux1 + po;
+                typename SA::st *rr = (typename SA::st *)ea.aux2 + po;
+                if (!(ea.flag & 1)) {
+                    cx<T> pa, pb;
+                    S::ld2((const typename S::st *)ea.aux0 + po, g, pa, pb);
+                    cx<TA> xa, xb;
+                    SA::ld2(xr, g, xa, xb);
+                    xa = {xa.r + alpha * (TA)pa.r, xa.i + alpha * (TA)pa.i};
+                    xb = {xb.r + alpha * (TA)pb.r, xb.i + alpha * (TA)pb.i};
+                    store_nt2<SA>(xr, g, SA::pack2(xa, xb));
+                }
+                cx<TA> ra, rb;
+                SA::ld2(rr, g, ra, rb);
+                ra = {ra.r - alpha * (TA)o0[d].r, ra.i - alpha * (TA)o0[d].i};
+                rb = {rb.r - alpha * (TA)o1[d].r, rb.i - alpha * (TA)o1[d].i};
+                store_nt2<SA>(rr, g, SA::pack2(ra, rb));
+                acc += ra.r * ra.r + ra.i * ra.i + rb.r * rb.r + rb.i * rb.i;
+            } else {
+                store_nt2<S>(out + po, g, S::pack2(o0[d], o1[d]));
+                if (EPI == EPI_NORM)
+                    acc += o0[d].r * o0[d].r + o0[d].i * o0[d].i + o1[d].r * o1[d].r + o1[d].i * o1[d].i;
+                if (EPI == EPI_DOT) {
+                    cx<T> a, b;
+                    S::ld2((const typename S::st *)ea.aux0 + po, g, a, b);
+                    acc += a.r * o0[d].r + a.i * o0[d].i + b.r * o1[d].r + b.i * o1[d].i;
+                }
+            }
+        }
+        part = (double)acc;
+    }
+    if (EPI == EPI_NORM || EPI == EPI_DOT || EPI == EPI_CG || EPI == EPI_CGQ) {
+        const double t = block_sum(part);
+        if (threadIdx.x == 0) ea.partial[lb] = t;
+    }
+}
+
+template <class S, class SA, bool DAG, int EPI>
+__global__ __launch_bounds__(kBlock) void k_dsl_pair(Lat lat, typename S::st *__restrict__ out,
+                                                     const typename S::st *__restrict__ in,
+                                                     const double *__restrict__ chi, double hpin, int imass,
+                                                     EpiArgs ea) {
+    if (ea.skip && *ea.skip) return;
+    dsl_block_pair<S, SA, DAG, EPI>(lat, out, in, chi, hpin, imass, ea, logical_block_x());
+}
+
+// Grid size (workgroups) of a k_dsl_pair launch.
+inline unsigned stencil_grid_pair(long n5) {
+    unsigned nb = 2 * stencil_pair_blocks_per_flavour(n5);
+    return (nb + 7u) / 8u * 8u;
 }
 
 // Grid size (workgroups) of a k_dsl launch.

@@ -116,3 +116,39 @@ def test_native_layout_round_trip(prec):
         assert np.array_equal(back, R.astype(np.complex64).astype(np.complex128))
     else:
         assert np.abs(back - R).max() <= 2 ** -8 * np.abs(R).max()
+
+
+@pytest.mark.parametrize("prec", ["fp32"])
+@pytest.mark.parametrize("L,Ls,Lt", [(2, 2, 2), (4, 3, 6), (8, 8, 8), (6, 5, 3), (12, 24, 12)])
+def test_pair_stencil_equals_one_site(prec, L, Ls, Lt, monkeypatch):
+    """The paired-site FP32 stencil (two x-neighbours per thread) gives
+    bit for bit the one-site stencil's output, with the wall terms (both
+    imass, hpin at site 0) and the antiperiodic t boundary."""
+    monkeypatch.setenv("CHS_PAIRS", "0")
+    one = fm.Lattice(L, Ls, Lt)
+    monkeypatch.setenv("CHS_PAIRS", "1")
+    two = fm.Lattice(L, Ls, Lt)
+    rs = np.random.RandomState(31 * L + Ls)
+    R, chi = spinor(rs, one.shape), phi_field(rs, one.kvol)
+    for imass in (1, 3):
+        for dagger in (False, True):
+            outs = []
+            for lat in (one, two):
+                chi_d = fm.DeviceArray.from_numpy(lat, chi)
+                rin = fm.spinor_dev(lat, R, prec)
+                out = fm.spinor_dev(lat, None, prec)
+                fm.dslash_dev(lat, out, rin, chi_d, 0.7, imass, dagger)
+                outs.append(fm.to_host(lat, out))
+            assert np.array_equal(outs[0], outs[1]), (prec, imass, dagger, np.abs(outs[0] - outs[1]).max())
+
+
+def test_odd_L_low_precision_uses_one_site_stencil():
+    lat, olat = lattices(5, 4, 6)
+    rs = np.random.RandomState(8)
+    R, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    chi_d = fm.DeviceArray.from_numpy(lat, chi)
+    rin = fm.spinor_dev(lat, R, "fp32")
+    out = fm.spinor_dev(lat, None, "fp32")
+    fm.dslash_dev(lat, out, rin, chi_d, 0.0, 1, False)
+    c = orc.dslash(olat, R, chi, 0.0, 1, False)
+    assert np.linalg.norm(fm.to_host(lat, out) - c) <= 2e-6 * np.linalg.norm(c)
