@@ -46,6 +46,7 @@ struct chs_ctx {
     struct NNState *nn = nullptr; // learned preconditioner (chs_learn_host.inc)
     int use_graphs = 1;           // replay solver iteration chunks from hipGraphs (CHS_GRAPHS=0: off)
     int pair_stencil = 1;         // paired-site FP32 stencil on even L (CHS_PAIRS=0: off)
+    int force_prec = CHS_FP64;    // storage of the force's multi-shift solves (chs_set_force_precision)
 };
 static void nn_destroy(chs_ctx *c);
 
@@ -413,21 +414,32 @@ int chs_dslashd(chs_ctx *c, double *phi, const double *r, const double *chi, dou
 }
 
 // ---- multi-shift Lanczos ------------------------------------------------------------
-int chs_qmrherm_dev(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass,
-                    const double *anum, const double *aden, int ndiag, int iflag, const double *chi, double *x,
-                    double *dsdchi) {
+}  // extern "C"
+
+namespace {
+
+// Multi-shift Lanczos (rhmc/cheisdwf.f:599-745) with the spinors r, q, w, v,
+// p_k (and x_k) stored in S.  Scalars, reductions and, for iflag = 0, the
+// running rational combination x are FP64 whatever S is; FP32 storage halves
+// the traffic (the force's solves; their error against FP64 is tested).
+template <class S>
+int qmrherm_impl(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass,
+                 const double *anum, const double *aden, int ndiag, int iflag, const double *chi, double *x,
+                 double *dsdchi) {
+    typedef typename S::st st;
     if (ndiag < 1 || ndiag > kMaxShift) return fail(c, CHS_EINVAL, "ndiag=%d outside 1..%d", ndiag, kMaxShift);
     if (iflag == 1 && !dsdchi) return fail(c, CHS_EINVAL, "iflag=1 needs dsdchi");
     const long n = c->kferm;
     int rc = ensure_spinors(c, {B_R, B_Q0, B_Q1, B_V, B_W});
     if (rc) return rc;
-    if ((rc = ensure(c, c->shifts_p, (size_t)ndiag * n * 16))) return rc;
+    if ((rc = ensure(c, c->shifts_p, (size_t)ndiag * n * sizeof(st)))) return rc;
     // iflag = 0 accumulates the rational combination directly into x; only
     // the force (iflag = 1) needs every x_k
-    if (iflag == 1 && (rc = ensure(c, c->shifts_x, (size_t)ndiag * n * 16))) return rc;
-    double2 *r = buf<double2>(c, B_R), *q0 = buf<double2>(c, B_Q0), *q1 = buf<double2>(c, B_Q1);
-    double2 *v = buf<double2>(c, B_V), *w = buf<double2>(c, B_W);
-    double2 *pk = (double2 *)c->shifts_p.p, *xk = iflag == 1 ? (double2 *)c->shifts_x.p : (double2 *)x;
+    if (iflag == 1 && (rc = ensure(c, c->shifts_x, (size_t)ndiag * n * sizeof(st)))) return rc;
+    st *r = buf<st>(c, B_R), *q0 = buf<st>(c, B_Q0), *q1 = buf<st>(c, B_Q1);
+    st *v = buf<st>(c, B_V), *w = buf<st>(c, B_W);
+    st *pk = (st *)c->shifts_p.p;
+    void *xk = iflag == 1 ? c->shifts_x.p : (void *)x;
 
     // scalar state: aden, ndiag; coefficients anum for the combine / force
     LzState h;
@@ -440,32 +452,36 @@ int chs_qmrherm_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
     // x = a0 Phi; r = Phi; q_{0} = 0; beta = phimod = |Phi|
     hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, anum[0], (const double2 *)phi,
                        0.0, (double2 *)x);
-    HIPCHK(c, hipMemcpyAsync(r, phi, 16 * n, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemsetAsync(q1, 0, 16 * n, c->stream));
+    if (std::is_same<S, F64>::value)
+        HIPCHK(c, hipMemcpyAsync(r, phi, 16 * n, hipMemcpyDeviceToDevice, c->stream));
+    else
+        hipLaunchKernelGGL((k_convert<S, F64>), dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, r,
+                           (const double2 *)phi);
+    HIPCHK(c, hipMemsetAsync(q1, 0, sizeof(st) * n, c->stream));
     reduce_launch(c, n, phi, nullptr, 0, c->dscal);
     hipLaunchKernelGGL(k_lz_init, dim3(1), dim3(kBlock), 0, c->stream, c->lz, (const double *)c->dscal, 1,
                        std::sqrt((double)n * res * res));
 
     const int niterc = 5000;
-    const int np1 = (int)stencil_grid(c->n5), np2 = (int)nblocks(n);
-    const int chunk = chunk_for(c, (12.0 + 4.0 * ndiag) * 16.0 * n);
+    const int np1 = (int)dsl_grid<S>(c), np2 = (int)nblocks(n);
+    const int chunk = chunk_for(c, (12.0 + 2.0 * ndiag) * (double)sizeof(st) * n);
     const int *skip = &c->lz->done;
     int done = 0, iter = 1;
     while (iter <= niterc && !done) {
         const int last = std::min(niterc, iter + chunk - 1);
         for (; iter <= last; iter++) {
-            double2 *qc = (iter & 1) ? q0 : q1, *qp = (iter & 1) ? q1 : q0;
-            dslash_launch<F64, F64, false, EPI_SCALEQ>(c, v, r, chi, hpin, imass,
-                                                      epi(nullptr, &c->lz->beta, qc, nullptr, nullptr, skip));
-            dslash_launch<F64, F64, true, EPI_DOT>(c, w, v, chi, hpin, imass,
-                                                  epi(c->partial, nullptr, qc, nullptr, nullptr, skip));
+            st *qc = (iter & 1) ? q0 : q1, *qp = (iter & 1) ? q1 : q0;
+            dslash_launch<S, S, false, EPI_SCALEQ>(c, v, r, chi, hpin, imass,
+                                                  epi(nullptr, &c->lz->beta, qc, nullptr, nullptr, skip));
+            dslash_launch<S, S, true, EPI_DOT>(c, w, v, chi, hpin, imass,
+                                              epi(c->partial, nullptr, qc, nullptr, nullptr, skip));
             hipLaunchKernelGGL(k_lz_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->lz, (const double *)c->partial,
                                np1, iter);
             const double *an = (const double *)c->dcoef;
             const LzState *lzc = c->lz;
-#define LZ_UPD(NS, CMB)                                                                                        \
-    hipLaunchKernelGGL((k_lz_update<NS, CMB>), dim3(np2), dim3(kBlock), 0, c->stream, n, (const double2 *)w,   \
-                       (const double2 *)qc, (const double2 *)qp, r, pk, xk, an, lzc, ndiag, iter, c->partial2)
+#define LZ_UPD(NS, CMB)                                                                                         \
+    hipLaunchKernelGGL((k_lz_update<S, NS, CMB>), dim3(np2), dim3(kBlock), 0, c->stream, n, (const st *)w,      \
+                       (const st *)qc, (const st *)qp, r, pk, xk, an, lzc, ndiag, iter, c->partial2)
             if (iflag == 0) {
                 if (ndiag == 25) LZ_UPD(25, true);
                 else if (ndiag == 12) LZ_UPD(12, true);
@@ -489,14 +505,30 @@ int chs_qmrherm_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
     if (iflag == 1) {
         const size_t cb = (size_t)ndiag * 3 * c->kvol * sizeof(double);
         if ((rc = ensure(c, c->contrib, cb))) return rc;
-        hipLaunchKernelGGL(k_force_shift, dim3(nblocks((long)c->kvol * ndiag)), dim3(kBlock), 0, c->stream, c->lat,
-                           (const double2 *)xk, n, ndiag, (const double *)c->dcoef, chi, hpin, imass,
+        hipLaunchKernelGGL((k_force_shift<S>), dim3(nblocks((long)c->kvol * ndiag)), dim3(kBlock), 0, c->stream,
+                           c->lat, (const st *)xk, n, ndiag, (const double *)c->dcoef, chi, hpin, imass,
                            (double *)c->contrib.p);
         hipLaunchKernelGGL(k_force_sum, dim3(nblocks(3 * c->kvol)), dim3(kBlock), 0, c->stream, (int)c->kvol,
                            ndiag, (const double *)c->contrib.p, dsdchi);
     }
     HIPCHK(c, hipGetLastError());
     // the reference returns silently after niterc iterations (rhmc/cheisdwf.f:639)
+    return CHS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chs_qmrherm_dev(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass,
+                    const double *anum, const double *aden, int ndiag, int iflag, const double *chi, double *x,
+                    double *dsdchi) {
+    return qmrherm_impl<F64>(c, phi, res, itercg, hpin, imass, anum, aden, ndiag, iflag, chi, x, dsdchi);
+}
+
+int chs_set_force_precision(chs_ctx *c, int prec) {
+    if (prec != CHS_FP64 && prec != CHS_FP32) return fail(c, CHS_EINVAL, "force precision must be FP64 or FP32");
+    c->force_prec = prec;
     return CHS_OK;
 }
 
@@ -611,10 +643,12 @@ int chs_force(chs_ctx *c, const double *phi, double res1, double am, double hpin
     int i1 = 0, i2 = 0;
     // X = r_b(M^dagger M) Phi with the remez2g b-set, then the force from the a-set
     // (rhmc/cheisdwf.f:500-513)
-    rc = chs_qmrherm_dev(c, dphi, res1, &i1, hpin, imass, bnum, bden, ndiag, 0, dchi, dx, nullptr);
+    // (solver storage per the context's force precision policy)
+    auto ms = c->force_prec == CHS_FP32 ? qmrherm_impl<F32> : qmrherm_impl<F64>;
+    rc = ms(c, dphi, res1, &i1, hpin, imass, bnum, bden, ndiag, 0, dchi, dx, nullptr);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(dx2, dx, 16 * c->kferm, hipMemcpyDeviceToDevice, c->stream));
-    rc = chs_qmrherm_dev(c, dx2, res1, &i2, hpin, imass, anum, aden, ndiag, 1, dchi, dx, dF);
+    rc = ms(c, dx2, res1, &i2, hpin, imass, anum, aden, ndiag, 1, dchi, dx, dF);
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(dsdchi, dF, 24 * c->kvol, hipMemcpyDeviceToHost, c->stream));
     if (x && (rc = d2h_spinor(c, x, dx))) return rc;

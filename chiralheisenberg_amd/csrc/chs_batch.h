@@ -96,11 +96,11 @@ __device__ __forceinline__ void ms_update(const MsSys &s, int parity, long lb) {
     const double2 *qp = parity ? s.q1 : s.q0;
     const int iter = s.lz->itercg + 1;
     if (s.ndiag == 25)
-        lz_update<25, true, EPT>(s.n, s.w, qc, qp, s.r, s.pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
+        lz_update<F64, 25, true, EPT>(s.n, s.w, qc, qp, s.r, s.pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
     else if (s.ndiag == 12)
-        lz_update<12, true, EPT>(s.n, s.w, qc, qp, s.r, s.pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
+        lz_update<F64, 12, true, EPT>(s.n, s.w, qc, qp, s.r, s.pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
     else
-        lz_update<0, true, EPT>(s.n, s.w, qc, qp, s.r, s.pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
+        lz_update<F64, 0, true, EPT>(s.n, s.w, qc, qp, s.r, s.pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
 }
 
 template <int EPT>

@@ -333,58 +333,68 @@ __device__ __forceinline__ void lz_coefs(LzCoef &sc, const LzState *__restrict__
     __syncthreads();
 }
 
-template <int NS, bool CMB>
-__device__ __forceinline__ double lz_update_elem(long i, long n, const double2 *__restrict__ w,
-                                                 const double2 *__restrict__ q, const double2 *__restrict__ qm1,
-                                                 double2 *__restrict__ r, double2 *__restrict__ p,
-                                                 double2 *__restrict__ x, const LzState *__restrict__ st,
+// Storage S of r, q, w, p_k (and x_k); arithmetic in FP64 on loaded values,
+// rounded on store.  With CMB the running combination x is FP64 (double2).
+template <class S, int NS, bool CMB>
+__device__ __forceinline__ double lz_update_elem(long i, long n, const typename S::st *__restrict__ w,
+                                                 const typename S::st *__restrict__ q,
+                                                 const typename S::st *__restrict__ qm1,
+                                                 typename S::st *__restrict__ r, typename S::st *__restrict__ p,
+                                                 void *__restrict__ xp, const LzState *__restrict__ st,
                                                  const LzCoef &sc, int ndiag, int iter) {
+    typedef cx<typename S::rt> C;
     const double at = st->alphatild, b0 = st->beta;
-    const double2 qv = q[i], wv = w[i];
-    const double2 qm = qm1[i];
+    const C qs = S::ld(q, i), ws = S::ld(w, i), qms = S::ld(qm1, i);
+    const double2 qv = make_double2(qs.r, qs.i), wv = make_double2(ws.r, ws.i), qm = make_double2(qms.r, qms.i);
     double2 rv;
     rv.x = wv.x - at * qv.x - b0 * qm.x;
     rv.y = wv.y - at * qv.y - b0 * qm.y;
-    r[i] = rv;
+    S::sto(r, i, C{(typename S::rt)rv.x, (typename S::rt)rv.y});
     const int ns = NS > 0 ? NS : ndiag;
     if (CMB) {
+        double2 *x = (double2 *)xp;
         double2 xv = x[i];
         if (iter == 1) {
 #pragma unroll 4
             for (int k = 0; k < ns; k++) {
-                p[(long)k * n + i] = qv;
+                S::sto(p, (long)k * n + i, qs);
                 xv.x += sc.c[k] * qv.x;
                 xv.y += sc.c[k] * qv.y;
             }
         } else {
 #pragma unroll 4
             for (int k = 0; k < ns; k++) {
-                double2 pv = p[(long)k * n + i];
-                pv.x = qv.x - sc.amu[k] * pv.x;
-                pv.y = qv.y - sc.amu[k] * pv.y;
-                p[(long)k * n + i] = pv;
+                const C ps = S::ld(p, (long)k * n + i);
+                double2 pv;
+                pv.x = qv.x - sc.amu[k] * (double)ps.r;
+                pv.y = qv.y - sc.amu[k] * (double)ps.i;
+                S::sto(p, (long)k * n + i, C{(typename S::rt)pv.x, (typename S::rt)pv.y});
                 xv.x += sc.c[k] * pv.x;
                 xv.y += sc.c[k] * pv.y;
             }
         }
         x[i] = xv;
-    } else if (iter == 1) {
-#pragma unroll 4
-        for (int k = 0; k < ns; k++) {
-            p[(long)k * n + i] = qv;
-            x[(long)k * n + i] = make_double2(sc.rho[k] * qv.x, sc.rho[k] * qv.y);
-        }
     } else {
+        typename S::st *x = (typename S::st *)xp;
+        if (iter == 1) {
 #pragma unroll 4
-        for (int k = 0; k < ns; k++) {
-            double2 pv = p[(long)k * n + i];
-            double2 xv = x[(long)k * n + i];
-            pv.x = qv.x - sc.amu[k] * pv.x;
-            pv.y = qv.y - sc.amu[k] * pv.y;
-            xv.x += sc.rho[k] * pv.x;
-            xv.y += sc.rho[k] * pv.y;
-            p[(long)k * n + i] = pv;
-            x[(long)k * n + i] = xv;
+            for (int k = 0; k < ns; k++) {
+                S::sto(p, (long)k * n + i, qs);
+                S::sto(x, (long)k * n + i,
+                       C{(typename S::rt)(sc.rho[k] * qv.x), (typename S::rt)(sc.rho[k] * qv.y)});
+            }
+        } else {
+#pragma unroll 4
+            for (int k = 0; k < ns; k++) {
+                const C ps = S::ld(p, (long)k * n + i), xs = S::ld(x, (long)k * n + i);
+                double2 pv, xv;
+                pv.x = qv.x - sc.amu[k] * (double)ps.r;
+                pv.y = qv.y - sc.amu[k] * (double)ps.i;
+                xv.x = (double)xs.r + sc.rho[k] * pv.x;
+                xv.y = (double)xs.i + sc.rho[k] * pv.y;
+                S::sto(p, (long)k * n + i, C{(typename S::rt)pv.x, (typename S::rt)pv.y});
+                S::sto(x, (long)k * n + i, C{(typename S::rt)xv.x, (typename S::rt)xv.y});
+            }
         }
     }
     return rv.x * rv.x + rv.y * rv.y;
@@ -393,10 +403,11 @@ __device__ __forceinline__ double lz_update_elem(long i, long n, const double2 *
 // EPT consecutive element blocks of kBlock per workgroup (block lb covers
 // element blocks lb*EPT .. lb*EPT+EPT-1), one |r|^2 partial per element block,
 // so the partials do not depend on EPT.
-template <int NS, bool CMB, int EPT>
-__device__ __forceinline__ void lz_update(long n, const double2 *__restrict__ w, const double2 *__restrict__ q,
-                                          const double2 *__restrict__ qm1, double2 *__restrict__ r,
-                                          double2 *__restrict__ p, double2 *__restrict__ x,
+template <class S, int NS, bool CMB, int EPT>
+__device__ __forceinline__ void lz_update(long n, const typename S::st *__restrict__ w,
+                                          const typename S::st *__restrict__ q,
+                                          const typename S::st *__restrict__ qm1, typename S::st *__restrict__ r,
+                                          typename S::st *__restrict__ p, void *__restrict__ x,
                                           const double *__restrict__ anum, const LzState *__restrict__ st, int ndiag,
                                           int iter, double *__restrict__ partial, long lb) {
     __shared__ LzCoef sc;
@@ -406,22 +417,22 @@ __device__ __forceinline__ void lz_update(long n, const double2 *__restrict__ w,
         const long eb = lb * EPT + e;
         const long i = eb * kBlock + threadIdx.x;
         double part = 0.0;
-        if (i < n) part = lz_update_elem<NS, CMB>(i, n, w, q, qm1, r, p, x, st, sc, ndiag, iter);
+        if (i < n) part = lz_update_elem<S, NS, CMB>(i, n, w, q, qm1, r, p, x, st, sc, ndiag, iter);
         const double t = block_sum(part);
         if (threadIdx.x == 0) partial[eb] = t;
     }
 }
 
-template <int NS, bool CMB>
-__global__ __launch_bounds__(kBlock) void k_lz_update(long n, const double2 *__restrict__ w,
-                                                      const double2 *__restrict__ q,
-                                                      const double2 *__restrict__ qm1, double2 *__restrict__ r,
-                                                      double2 *__restrict__ p, double2 *__restrict__ x,
-                                                      const double *__restrict__ anum,
+template <class S, int NS, bool CMB>
+__global__ __launch_bounds__(kBlock) void k_lz_update(long n, const typename S::st *__restrict__ w,
+                                                      const typename S::st *__restrict__ q,
+                                                      const typename S::st *__restrict__ qm1,
+                                                      typename S::st *__restrict__ r, typename S::st *__restrict__ p,
+                                                      void *__restrict__ x, const double *__restrict__ anum,
                                                       const LzState *__restrict__ st, int ndiag, int iter,
                                                       double *__restrict__ partial) {
     if (st->done) return;
-    lz_update<NS, CMB, 1>(n, w, q, qm1, r, p, x, anum, st, ndiag, iter, partial, logical_block());
+    lz_update<S, NS, CMB, 1>(n, w, q, qm1, r, p, x, anum, st, ndiag, iter, partial, logical_block());
 }
 
 // x = a0 phi + sum_k a_k x_k, summed in shift order (rhmc/cheisdwf.f:722-727)

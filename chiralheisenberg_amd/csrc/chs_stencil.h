@@ -493,28 +493,31 @@ inline unsigned stencil_grid(long n5) {
 // One thread per (site, shift): y = x_k, z = M x_k on the slices s = 0 and
 // s = Ls-1 only (derivs uses no other slice), contribution of shift k at the
 // site into contrib[(k*3 + a)*kvol + site].  Native layout.
-__global__ __launch_bounds__(kBlock) void k_force_shift(Lat lat, const double2 *__restrict__ xk, long n,
+// (x_k stored in S; the bilinears and their sums are FP64)
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_force_shift(Lat lat, const typename S::st *__restrict__ xk, long n,
                                                         int ndiag, const double *__restrict__ anum,
                                                         const double *__restrict__ chi, double hpin, int imass,
                                                         double *__restrict__ contrib) {
+    typedef typename S::rt T;
     const long gid = (long)blockIdx.x * kBlock + threadIdx.x;
     if (gid >= (long)lat.kvol * ndiag) return;
     const int k = (int)(gid / lat.kvol);
     const unsigned site = (unsigned)(gid - (long)k * lat.kvol);
-    const double2 *y = xk + (size_t)k * n;
-    cx<double> zf[2][4], cf[2][4], zl[2][4], cl[2][4];
-    stencil<F64, false, 2, 1>(lat, y, chi, hpin, imass, 0, site, 0, zf, cf);
-    stencil<F64, false, 2, 1>(lat, y, chi, hpin, imass, (unsigned)lat.Ls - 1, site, 0, zl, cl);
+    const typename S::st *y = xk + (size_t)k * n;
+    cx<T> zf[2][4], cf[2][4], zl[2][4], cl[2][4];
+    stencil<S, false, 2, 1>(lat, y, chi, hpin, imass, 0, site, 0, zf, cf);
+    stencil<S, false, 2, 1>(lat, y, chi, hpin, imass, (unsigned)lat.Ls - 1, site, 0, zl, cl);
     double2 ya[4][2], za[4][2];
 #pragma unroll
     for (int f = 0; f < 2; f++)
 #pragma unroll
         for (int d = 0; d < 4; d++) {
             // d = 0,1: y(s=Ls) with z(s=1); d = 2,3: y(s=1) with z(s=Ls)
-            const cx<double> yv = d < 2 ? cl[f][d] : cf[f][d];
-            const cx<double> zv = d < 2 ? zf[f][d] : zl[f][d];
-            ya[d][f] = make_double2(yv.r, yv.i);
-            za[d][f] = make_double2(zv.r, zv.i);
+            const cx<T> yv = d < 2 ? cl[f][d] : cf[f][d];
+            const cx<T> zv = d < 2 ? zf[f][d] : zl[f][d];
+            ya[d][f] = make_double2((double)yv.r, (double)yv.i);
+            za[d][f] = make_double2((double)zv.r, (double)zv.i);
         }
     double F1 = 0, F2 = 0, F3 = 0;
     derivs_site(F1, F2, F3, ya, za, -2.0 * anum[k + 1], imass);

@@ -148,6 +148,12 @@ def force(lat, Phi, res1, chi, beta, remez2g, am=0.0, hpin=0.0, imass=1):
     return F, x.reshape(lat.shape), (its[0], its[1])
 
 
+def set_force_precision(lat, precision):
+    """Storage precision ("fp64" or "fp32") of the multi-shift solves inside
+    force() for this lattice context; hamilton stays FP64."""
+    check(lat.h, lib().chs_set_force_precision(lat.h, PRECISIONS[precision]))
+
+
 def hamilton(lat, Phi, res2, chi, pp, beta, remez2, am=0.0, hpin=0.0, imass=1, isweep=1):
     """H = beta sum chi^2 + 1/2 sum p^2 + Re Phi^dagger r_b(M^dagger M) Phi
     (rhmc/cheisdwf.f:522-589).  Returns ((h, hg, hp, s), x, itercg)."""

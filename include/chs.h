@@ -105,6 +105,13 @@ int chs_force(chs_ctx *ctx, const double *phi, double res1, double am,
               const double *bden, int ndiag, double *dsdchi, double *x,
               int *its);
 
+/* Storage precision of the two multi-shift solves inside chs_force
+ * (CHS_FP64, the default, or CHS_FP32: spinors in FP32, Lanczos scalars,
+ * reductions and the derivs bilinears and sums in FP64).  No counterpart in
+ * the reference (which computes in FP64 throughout); hamilton, and with it
+ * the Metropolis test, stays FP64. */
+int chs_set_force_precision(chs_ctx *ctx, int prec);
+
 /* H = beta sum chi^2 + 1/2 sum p^2 + Re Phi^dagger (M^dagger M)^{-1/2} Phi.
  * Replaces hamilton(Phi,h,hg,hp,s,res2,isweep,am,hpin,imass),
  * rhmc/cheisdwf.f:522, with /trial/ chit, pp and /remez2/ b-set explicit.

@@ -45,3 +45,32 @@ def test_trajectories_match_oracle(L, Ls, ntraj, imass, dt):
     assert np.abs(chi_g - chi_c).max() <= 1e-9 * np.abs(chi_c).max()
     assert rng.seed() == orng.get()
     lat.close()
+
+
+def test_trajectories_fp32_force_solves():
+    """Production tolerances (1e-9 action, 1e-6 force) with the force's
+    multi-shift solves in FP32 storage against the FP64 oracle: the same
+    trajectory lengths and accept/reject decisions; the dH difference is
+    stated and bounded."""
+    L, Ls, ntraj = 4, 4, 8
+    par = dict(dt=0.45, beta=0.475, hpin=0.0, imass=1, iterl=3, itermax=1000, iprint=2,
+               rescga=1e-9, rescgg=1e-6, am=0.0)
+    sets = {k: remez.load(k) for k in ("remez2", "remez4", "remez2g")}
+    rs = np.random.RandomState(104)
+    chi0 = rs.randn(3, L * L * L) / np.sqrt(0.475)
+    seed = 693764523745.0
+    lat = fm.Lattice(L, Ls)
+    fm.set_force_precision(lat, "fp32")
+    chi_g = chi0.copy()
+    rng = rhmc.RandomStream(seed)
+    gpu = [rhmc.trajectory(lat, chi_g, rng, isweep=i + 1, par=par, sets=sets) for i in range(ntraj)]
+    olat = orc.Lattice(L, Ls)
+    chi_c = chi0.copy()
+    cpu = orc.rhmc(olat, par, orc.Rng(seed), chi_c, 1, ntraj, sets)
+    ddh = [abs(g["dH"] - c["dH"]) for g, c in zip(gpu, cpu)]
+    print("accepted %s; |dH(fp32 force) - dH(fp64 oracle)| max %.2e; dH %s" % (
+        [g["accepted"] for g in gpu], max(ddh), [round(c["dH"], 4) for c in cpu]))
+    assert [g["md_steps"] for g in gpu] == [c["md_steps"] for c in cpu]
+    assert [g["accepted"] for g in gpu] == [c["accepted"] for c in cpu]
+    assert max(ddh) <= 1e-4
+    lat.close()

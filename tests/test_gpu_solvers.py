@@ -80,6 +80,29 @@ def test_force_parity(imass):
     assert err < 1e-10, (err, itg, itc)
 
 
+@pytest.mark.parametrize("imass", [1, 3])
+def test_force_fp32_solves(imass):
+    """Force with FP32-storage multi-shift solves (FP64 Lanczos scalars,
+    reductions, derivs bilinears and sums) against the FP64 oracle at the
+    reference's force tolerance 1e-6 and at 1e-9: the error of the fermion
+    part is stated and bounded by 1e-4 relative (measured 1.4e-5 at 1e-6,
+    L=8, Ls=8, imass=1)."""
+    lat = fm.Lattice(8, 8)
+    olat = orc.Lattice(8, 8)
+    rs = np.random.RandomState(71 + imass)
+    phi, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    fm.set_force_precision(lat, "fp32")
+    for res in (1e-6, 1e-9):
+        Fg, _, itg = fm.force(lat, phi, res, chi, 0.475, R2G, 0.0, 0.2, imass)
+        Fc, _, itc = orc.force(olat, phi, res, chi, 0.475, R2G, 0.2, imass)
+        fg, fc = Fg - 2 * 0.475 * chi, Fc - 2 * 0.475 * chi
+        err = np.linalg.norm(fg - fc) / np.linalg.norm(fc)
+        print("imass %d res %g: FP32-solve force error %.2e (iterations %s vs %s)" % (imass, res, err, itg, itc))
+        assert err < 1e-4, err
+        assert abs(itg[0] - itc[0]) <= 0.3 * itc[0] + 5
+    lat.close()
+
+
 def test_hamilton_parity():
     lat, olat = lattices(8, 8)
     rs = np.random.RandomState(61)
