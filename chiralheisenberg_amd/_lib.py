@@ -73,6 +73,7 @@ def _declare(L):
         "chs_hamilton": ([v, v, v, v, v, v, d, i, d, d, i, v, v, d, v, v, i, v, ip], i),
         "chs_derivs": ([v, v, v, d, i, v], i),
         "chs_set_force_precision": ([v, i], i),
+        "chs_fermion": ([v, v, d, d, i, i, i, v, v, ip], i),
         "chs_alloc": ([v, sz], v),
         "chs_free": ([v, v], i),
         "chs_upload": ([v, v, v, sz], i),

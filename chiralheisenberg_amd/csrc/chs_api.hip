@@ -608,6 +608,92 @@ int chs_congrad_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
     return CHS_OK;
 }
 
+// fermion (rhmc/cheisdwf.f:897-1058): five wall sources at t-slices
+// mod(ittt0 + k (Lt/5), Lt) + 1, Dirac 3 and 4, flavour 1, 1/L^2 at s = 1;
+// x = (M^dagger M)^{-1} M^dagger source from the initial guess M^dagger
+// source (congrad, or the FP32 reliable-update solve to the same residual);
+// correlators from the t-slice sums of x at s = Ls (cferm1, Dirac = source)
+// and s = 1 (cferm2, gamma_3 partner), with the antiperiodic sign for slices
+// behind the source.
+int chs_fermion(chs_ctx *c, const double *chi, double res, double hpin, int imass, int ittt0, int prec,
+                double *cferm1, double *cferm2, int *iterations) {
+    if (ittt0 < 1 || ittt0 > c->Lt) return fail(c, CHS_EINVAL, "ittt0=%d outside 1..%d", ittt0, c->Lt);
+    if (prec != CHS_FP64 && prec != CHS_FP32) return fail(c, CHS_EINVAL, "fermion precision must be FP64 or FP32");
+    const int nsource = 5, Lt = c->Lt, Ls = c->Ls;
+    const long n = c->kferm, L2 = (long)c->L * c->L;
+    int rc = ensure_spinors(c, {B_PHI, B_X, B_Q0});
+    if (rc) return rc;
+    if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
+    if ((rc = ensure(c, c->b[B_PP], std::max<size_t>(24 * c->kvol, 64 * (size_t)Lt)))) return rc;
+    double2 *xi = buf<double2>(c, B_Q0), *phi = buf<double2>(c, B_PHI), *x = buf<double2>(c, B_X);
+    double2 *sums = buf<double2>(c, B_PP);
+    double *dchi = buf<double>(c, B_CHI);
+    HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    std::vector<double> c1(4 * Lt, 0.0), c2(4 * Lt, 0.0);   // [f][t] (re, im)
+    std::vector<double2> S[2];
+    int iter = 0, status = CHS_OK;
+    for (int ks = 1; ks <= nsource; ks++) {
+        const int ittt = (ittt0 + ks * (Lt / nsource)) % Lt + 1;
+        for (int ids = 0; ids < 2; ids++) {
+            HIPCHK(c, hipMemsetAsync(xi, 0, 16 * n, c->stream));
+            hipLaunchKernelGGL(k_wall_source, dim3((unsigned)((L2 + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               c->stream, c->lat, xi, ittt - 1, 2 + ids, 1.0 / (double)L2);
+            dslash_prec<true>(c, phi, xi, dchi, hpin, imass, CHS_FP64);
+            HIPCHK(c, hipMemcpyAsync(x, phi, 16 * n, hipMemcpyDeviceToDevice, c->stream));
+            int it = 0;
+            if (prec == CHS_FP64) {
+                rc = chs_congrad_dev(c, (const double *)phi, res, &it, hpin, imass, dchi, (double *)x);
+            } else {
+                // the reference's stopping rule |r|^2 < kferm res^2 as a relative tolerance
+                reduce_launch(c, n, phi, nullptr, 0, c->dscal);
+                double b2 = 0.0;
+                HIPCHK(c, hipMemcpyAsync(&b2, c->dscal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                chs_solve_stats st;
+                rc = chs_solve_mdagm_dev(c, (const double *)phi, (double *)x, dchi, hpin, imass,
+                                         std::sqrt((double)n * res * res / b2), CHS_FP32, 0.0, 0, &st);
+                it = st.iterations;
+            }
+            if (rc == CHS_ENOCONV) status = rc;   // the reference carries on with the last iterate
+            else if (rc) return rc;
+            iter += it;
+            SliceReq rq;
+            rq.s[0] = Ls - 1, rq.plane[0] = 2 + ids;       // cferm1, flavour 1
+            rq.s[1] = Ls - 1, rq.plane[1] = 2 + ids + 4;   // cferm1, flavour 2
+            rq.s[2] = 0, rq.plane[2] = ids;                 // cferm2 (gamin(3, 3 + ids) = 1 + ids)
+            rq.s[3] = 0, rq.plane[3] = ids + 4;
+            hipLaunchKernelGGL(k_slice_sums, dim3(Lt, 4), dim3(kBlock), 0, c->stream, c->lat, (const double2 *)x,
+                               rq, sums);
+            S[ids].resize(4 * Lt);
+            HIPCHK(c, hipMemcpyAsync(S[ids].data(), sums, 16 * 4 * Lt, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        // cferm1 += sign akappa prop0L(idd, idd); cferm2 += sign gamval(3, idd)
+        // prop00(idd, gamin(3, idd)), gamval(3, 3) = i/2, gamval(3, 4) = -i/2
+        for (int idd = 0; idd < 2; idd++) {
+            const double gi = idd == 0 ? 0.5 : -0.5;
+            for (int it = 0; it < Lt; it++) {
+                const int itt = (ittt + it - 1) % Lt + 1;
+                const double sgn = itt >= ittt ? 1.0 : (double)c->ibound;
+                for (int f = 0; f < 2; f++) {
+                    const double2 a = S[idd][(size_t)f * Lt + itt - 1], b = S[idd][(size_t)(2 + f) * Lt + itt - 1];
+                    double *p1 = &c1[2 * (it + (size_t)Lt * f)], *p2 = &c2[2 * (it + (size_t)Lt * f)];
+                    p1[0] += sgn * 0.5 * a.x;
+                    p1[1] += sgn * 0.5 * a.y;
+                    p2[0] += sgn * (-gi * b.y);
+                    p2[1] += sgn * (gi * b.x);
+                }
+            }
+        }
+    }
+    for (int k = 0; k < 4 * Lt; k++) {
+        cferm1[k] = c1[k] / nsource;
+        cferm2[k] = c2[k] / nsource;
+    }
+    if (iterations) *iterations = iter;
+    return status;
+}
+
 int chs_congrad(chs_ctx *c, const double *phi, double res, int *itercg, double am, double hpin, int imass,
                 const double *chi, double *x) {
     (void)am;

@@ -435,6 +435,37 @@ __global__ __launch_bounds__(kBlock) void k_lz_update(long n, const typename S::
     lz_update<S, NS, CMB, 1>(n, w, q, qm1, r, p, x, anum, st, ndiag, iter, partial, logical_block());
 }
 
+// ---- propagator measurement (rhmc/cheisdwf.f:897-1058) -------------------
+// Wall source: element (s = 0, every site of t-slice t0) of plane p set to val
+// in a native FP64 spinor (the rest was cleared by the caller).
+__global__ __launch_bounds__(kBlock) void k_wall_source(Lat lat, double2 *__restrict__ xi, int t0, int plane,
+                                                        double val) {
+    const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+    const long L2 = (long)lat.L * lat.L;
+    if (i < L2) xi[(long)plane * lat.ps + (long)t0 * L2 + i] = make_double2(val, 0.0);
+}
+
+// Sums over the L^2 sites of each t-slice for up to four (s, plane) pairs of
+// a native FP64 spinor: out[q * Lt + t], one workgroup per (t, q), fixed order.
+struct SliceReq {
+    int s[4], plane[4];
+};
+__global__ __launch_bounds__(kBlock) void k_slice_sums(Lat lat, const double2 *__restrict__ x, SliceReq rq,
+                                                       double2 *__restrict__ out) {
+    const int t = blockIdx.x, q = blockIdx.y;
+    const long L2 = (long)lat.L * lat.L;
+    const double2 *base = x + (long)rq.plane[q] * lat.ps + (long)rq.s[q] * lat.kvol + (long)t * L2;
+    double re = 0.0, im = 0.0;
+    for (long i = threadIdx.x; i < L2; i += kBlock) {
+        const double2 v = base[i];
+        re += v.x;
+        im += v.y;
+    }
+    re = block_sum(re);
+    im = block_sum(im);
+    if (threadIdx.x == 0) out[(long)q * lat.Lt + t] = make_double2(re, im);
+}
+
 // x = a0 phi + sum_k a_k x_k, summed in shift order (rhmc/cheisdwf.f:722-727)
 __global__ __launch_bounds__(kBlock) void k_lz_combine(long n, const double2 *__restrict__ phi,
                                                        const double2 *__restrict__ xk, int ndiag,

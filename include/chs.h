@@ -123,6 +123,18 @@ int chs_hamilton(chs_ctx *ctx, const double *phi, double *h, double *hg,
                  double beta, const double *bnum, const double *bden,
                  int ndiag, double *x, int *itercg);
 
+/* Wall-source fermion correlators on the domain walls.  Replaces
+ * fermion(res,aviter,am,hpin,imass), rhmc/cheisdwf.f:897-1058, with /trial/
+ * chi explicit and ittt0 = int(Lt*rano(yran,idum)) + 1 (the one random draw
+ * of the routine, 1..Lt) passed in.  Five wall sources, Dirac 3 and 4,
+ * congrad each (prec = CHS_FP64: the reference CGNR and stopping rule;
+ * CHS_FP32: the FP32 reliable-update solve to the same residual).
+ * cferm1 = fort.5001/5002, cferm2 = fort.5011/5012: complex (re, im)
+ * [flavour][t], t fastest, 2*Lt complex each.  iterations = total solver
+ * iterations (the reference's aviter times 10). */
+int chs_fermion(chs_ctx *ctx, const double *chi, double res, double hpin, int imass, int ittt0, int prec,
+                double *cferm1, double *cferm2, int *iterations);
+
 /* dSdchi += 2 a wall bilinears of r and x2 = M r.
  * Replaces derivs(R,X2,anum,imass), rhmc/cheisdwf.f:747. */
 int chs_derivs(chs_ctx *ctx, const double *r, const double *x2, double anum,

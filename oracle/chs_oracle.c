@@ -434,6 +434,66 @@ void orc_force(const orc_lat *g, const double *phi, double res1,
     free(x2);
 }
 
+/* rhmc/cheisdwf.f:897-1058.  Spinor element (s, site, d, f) of the reference
+ * layout: s + Ls*(site + kvol*(d + 4 f)). */
+int orc_fermion(const orc_lat *g, const double *chi, double res, double hpin,
+                int imass, int ittt0, double *cferm1, double *cferm2) {
+    const int nsource = 5;
+    const long kvol = kvol_of(g), n = kferm_of(g), L2 = (long)g->L * g->L;
+    const int Lt = g->Lt, Ls = g->Ls;
+    double *xi = (double *)calloc(2 * n, sizeof(double));
+    double *phi = (double *)malloc(sizeof(double) * 2 * n);
+    double *x[2];   /* solutions for Dirac source 3 and 4 */
+    x[0] = (double *)malloc(sizeof(double) * 2 * n);
+    x[1] = (double *)malloc(sizeof(double) * 2 * n);
+    memset(cferm1, 0, sizeof(double) * 2 * 2 * Lt);
+    memset(cferm2, 0, sizeof(double) * 2 * 2 * Lt);
+    int iter = 0;
+#define EL(s, site, d, f) ((long)(s) + Ls * ((long)(site) + kvol * ((long)(d) + 4L * (f))))
+    for (int ks = 1; ks <= nsource; ks++) {
+        const int ittt = (ittt0 + ks * (Lt / nsource)) % Lt + 1;   /* 1-based */
+        for (int ids = 0; ids < 2; ids++) {                       /* Dirac source 3, 4 */
+            memset(xi, 0, sizeof(double) * 2 * n);
+            for (long i = 0; i < L2; i++) xi[2 * EL(0, (ittt - 1) * L2 + i, 2 + ids, 0)] = 1.0 / (double)L2;
+            orc_dslash(g, phi, xi, chi, hpin, imass, 1);
+            memcpy(x[ids], phi, sizeof(double) * 2 * n);
+            iter += orc_congrad(g, phi, res, chi, hpin, imass, x[ids]);
+        }
+        /* cferm1 += sign 0.5 prop0L(idd, idd); cferm2 += sign gamval(3,idd)
+         * prop00(idd, gamin(3,idd)) with gamval(3,3) = 0.5 i, gamin(3,3) = 1,
+         * gamval(3,4) = -0.5 i, gamin(3,4) = 2 (rhmc/cheisdwf.f:1017-1036,
+         * :1555-1563, scaled by akappa = 0.5 at :1591-1595) */
+        for (int idd = 0; idd < 2; idd++) {
+            const double gi = idd == 0 ? 0.5 : -0.5;   /* gamval = gi * i */
+            const int dsink2 = idd;                     /* gamin - 1 */
+            for (int it = 0; it < Lt; it++) {
+                const int itt = (ittt + it - 1) % Lt + 1;
+                const double sgn = itt >= ittt ? 1.0 : (double)g->ibound;
+                const long ioff = (long)(itt - 1) * L2;
+                for (int f = 0; f < 2; f++) {
+                    double *c1 = cferm1 + 2 * (it + (long)Lt * f), *c2 = cferm2 + 2 * (it + (long)Lt * f);
+                    for (long i = 0; i < L2; i++) {
+                        const double *pL = x[idd] + 2 * EL(Ls - 1, ioff + i, 2 + idd, f);
+                        c1[0] += sgn * 0.5 * pL[0];
+                        c1[1] += sgn * 0.5 * pL[1];
+                        const double *p0 = x[idd] + 2 * EL(0, ioff + i, dsink2, f);
+                        /* (gi i) (a + i b) = -gi b + i gi a */
+                        c2[0] += sgn * (-gi * p0[1]);
+                        c2[1] += sgn * (gi * p0[0]);
+                    }
+                }
+            }
+        }
+    }
+#undef EL
+    for (int k = 0; k < 2 * 2 * Lt; k++) {
+        cferm1[k] /= nsource;
+        cferm2[k] /= nsource;
+    }
+    free(xi); free(phi); free(x[0]); free(x[1]);
+    return iter;
+}
+
 int orc_hamilton(const orc_lat *g, const double *phi, double res2,
                  const double *chi, const double *pp, double beta,
                  const double *bnum, const double *bden, int ndiag,

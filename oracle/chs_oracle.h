@@ -74,6 +74,17 @@ int orc_hamilton(const orc_lat *g, const double *phi, double res2,
                  const double *bnum, const double *bden, int ndiag,
                  double hpin, int imass, double *out, double *x);
 
+/* fermion (rhmc/cheisdwf.f:897-1058): wall-source propagators on the domain
+ * walls.  nsource = 5 wall sources at t-slices (1-based)
+ * mod(ittt0 + k*(Lt/5), Lt) + 1, k = 1..5, ittt0 in 1..Lt being the caller's
+ * draw int(Lt*rano) + 1; Dirac source 3 and 4, flavour 1, value 1/L^2 at
+ * s = 1; x = (M^dagger M)^{-1} M^dagger source by congrad from the initial
+ * guess M^dagger source.  cferm1, cferm2: complex (re, im) [f][t], t fastest,
+ * f = 0, 1 (the reference's fort.5001/5002 and fort.5011/5012).  Returns the
+ * total congrad iterations (the reference's aviter is that / 10). */
+int orc_fermion(const orc_lat *g, const double *chi, double res, double hpin,
+                int imass, int ittt0, double *cferm1, double *cferm2);
+
 /* ---- random numbers (rhmc/cheisdwf.f:1678-1801) ---------------------- */
 typedef struct {
     uint64_t s;       /* 48-bit LCG state */

@@ -68,6 +68,8 @@ def lib():
         _LIB.orc_force.argtypes = [L, p, d, p, d, p, p, p, p, i, d, i, p, p, p]
         _LIB.orc_hamilton.argtypes = [L, p, d, p, p, d, p, p, i, d, i, p, p]
         _LIB.orc_hamilton.restype = i
+        _LIB.orc_fermion.argtypes = [L, p, d, d, i, i, p, p]
+        _LIB.orc_fermion.restype = i
         R = ctypes.POINTER(OrcRng)
         _LIB.orc_rng_set.argtypes = [R, d]
         _LIB.orc_rng_get.argtypes = [R]
@@ -154,6 +156,18 @@ def congrad(lat, phi, res, chi, x0=None, hpin=0.0, imass=1):
     chi = np.ascontiguousarray(chi, dtype=np.float64)
     it = lib().orc_congrad(ctypes.byref(lat.c), _ptr(phi), float(res), _ptr(chi), float(hpin), imass, _ptr(x))
     return x, it
+
+
+def fermion(lat, chi, res, ittt0, hpin=0.0, imass=1):
+    """Wall-source correlators (rhmc/cheisdwf.f:897-1058) for the first source
+    slice ittt0 (1-based, the reference's int(Lt*rano)+1).  Returns (cferm1,
+    cferm2) complex (2 flavours, Lt) and the total congrad iterations."""
+    chi = np.ascontiguousarray(chi, dtype=np.float64)
+    c1 = np.zeros((2, lat.c.Lt), dtype=np.complex128)
+    c2 = np.zeros_like(c1)
+    it = lib().orc_fermion(ctypes.byref(lat.c), _ptr(chi), float(res), float(hpin), imass, int(ittt0), _ptr(c1),
+                           _ptr(c2))
+    return c1, c2, it
 
 
 def cg_iterations(lat, phi, chi, niter, hpin=0.0, imass=1):

@@ -159,6 +159,45 @@ class Ref:
         _run_big_stack(self.lib.hamilton_, *args)
         return h.value, hg.value, hp.value, s.value
 
+    def set_dirac(self):
+        """COMMON /dirac/ gamval(6,4), gamin(6,4) as init sets them
+        (rhmc/cheisdwf.f:1519-1595, values times akappa = 0.5)."""
+        one, zi = 1.0 + 0j, 1j
+        val = {1: [-zi, -zi, zi, zi], 2: [-one, one, one, -one], 3: [-zi, zi, zi, -zi], 4: [one, one, -one, -one],
+               5: [-one, -one, -one, -one], 6: [-one, -one, one, one]}
+        col = {1: [4, 3, 2, 1], 2: [4, 3, 2, 1], 3: [3, 4, 1, 2], 4: [1, 2, 3, 4], 5: [3, 4, 1, 2], 6: [3, 4, 1, 2]}
+        blk = (ctypes.c_char * (6 * 4 * 16 + 6 * 4 * 4)).in_dll(self.lib, "dirac_")
+        base = ctypes.addressof(blk)
+        gv = np.ctypeslib.as_array((ctypes.c_double * 48).from_address(base)).view(np.complex128)
+        gi = np.ctypeslib.as_array((ctypes.c_int * 24).from_address(base + 384))
+        for mu in range(1, 7):
+            for d in range(1, 5):
+                gv[(mu - 1) + 6 * (d - 1)] = 0.5 * val[mu][d - 1]
+                gi[(mu - 1) + 6 * (d - 1)] = col[mu][d - 1]
+
+    def fermion(self, chi, res, workdir, hpin=0.0, imass=1, am=0.0):
+        """The reference's own fermion(res,aviter,am,hpin,imass), run in
+        workdir (it writes fort.5001/5011/5002/5012 there).  /trial/ chi,
+        /para/ and /ranseed/ must be set.  Returns (cferm1, cferm2) complex
+        (2 flavours, Lt) and aviter."""
+        self.chit[:] = np.asarray(chi, dtype=np.float64)
+        self.set_dirac()
+        av = self.real(0)
+        old = os.getcwd()
+        os.chdir(workdir)
+        try:
+            _run_big_stack(self.lib.fermion_, self._r(res), ctypes.byref(av), self._r(am), self._r(hpin),
+                           ctypes.byref(ctypes.c_int(imass)))
+            out = []
+            for unit in (5001, 5011, 5002, 5012):
+                rows = np.loadtxt(os.path.join(workdir, "fort.%d" % unit))
+                out.append(rows[:, 1] + 1j * rows[:, 2])
+        finally:
+            os.chdir(old)
+        c1 = np.stack([out[0], out[2]])
+        c2 = np.stack([out[1], out[3]])
+        return c1, c2, av.value
+
     # random numbers --------------------------------------------------------
     def rranset(self, seed):
         self.lib.rranset_(ctypes.byref(ctypes.c_double(seed)))

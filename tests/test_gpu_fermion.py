@@ -1,0 +1,68 @@
+"""Wall-source fermion correlators (rhmc/cheisdwf.f:897-1058) on the GPU:
+against the reference's own routine (golden vectors on its thermalised
+configuration rhmc/_con, L=12, Ls=8) and against the oracle at other sizes;
+the FP32 reliable-update solves to the same stopping residual."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import phi_field
+
+pytestmark = pytest.mark.gpu
+
+fm = pytest.importorskip("chiralheisenberg_amd.fermion")
+from chiralheisenberg_amd import checkpoint  # noqa: E402
+from chiralheisenberg_amd._lib import ChsError  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_fermion_vs_reference_golden():
+    g = np.load(os.path.join(HERE, "golden", "ref_fermion_l12_ls8.npz"))
+    chi, _, _ = checkpoint.read_con(os.path.join(HERE, "golden", "con_l12_fp64"), kvol=1728)
+    lat = fm.Lattice(12, 8)
+    for k, tol in ((0, 5e-9), (1, 1e-5)):
+        rng = orc.Rng(float(g["rng_seed"]))   # the reference generator, restated
+        rng.rano(0)
+        ittt0 = int(12 * rng.rano(0)) + 1
+        c1, c2, it = fm.fermion(lat, chi, float(g["case%d_res" % k]), ittt0, float(g["case%d_hpin" % k]),
+                                int(g["case%d_imass" % k]))
+        r1, r2 = g["case%d_cferm1" % k], g["case%d_cferm2" % k]
+        e1 = np.abs(c1 - r1).max() / np.abs(r1).max()
+        e2 = np.abs(c2 - r2).max() / np.abs(r2).max()
+        print("case %d: cferm1 %.2e cferm2 %.2e, aviter %.1f vs %.1f" % (k, e1, e2, it / 10,
+                                                                          float(g["case%d_aviter" % k])))
+        assert e1 <= tol and e2 <= tol
+        assert abs(it / 10 - float(g["case%d_aviter" % k])) <= 3
+    lat.close()
+
+
+@pytest.mark.parametrize("L,Ls,Lt,imass,ittt0", [(6, 4, 10, 1, 3), (8, 6, 8, 3, 8)])
+def test_fermion_vs_oracle(L, Ls, Lt, imass, ittt0):
+    lat = fm.Lattice(L, Ls, Lt)
+    olat = orc.Lattice(L, Ls, Lt)
+    rs = np.random.RandomState(L * 10 + Ls)
+    chi = phi_field(rs, lat.kvol)
+    c1, c2, it = fm.fermion(lat, chi, 1e-11, ittt0, 0.3, imass)
+    o1, o2, oit = orc.fermion(olat, chi, 1e-11, ittt0, 0.3, imass)
+    assert np.abs(c1 - o1).max() <= 1e-8 * np.abs(o1).max()
+    assert np.abs(c2 - o2).max() <= 1e-8 * np.abs(o2).max()
+    assert abs(it - oit) <= 20
+    # FP32 reliable-update solves to the same stopping residual
+    f1, f2, fit = fm.fermion(lat, chi, 1e-11, ittt0, 0.3, imass, precision="fp32")
+    e = max(np.abs(f1 - o1).max() / np.abs(o1).max(), np.abs(f2 - o2).max() / np.abs(o2).max())
+    print("L=%d Ls=%d: fp32 correlator error %.2e, iterations %d vs %d" % (L, Ls, e, fit, it))
+    assert e <= 1e-6
+    lat.close()
+
+
+def test_fermion_rejects_bad_source_slice():
+    lat = fm.Lattice(4, 4)
+    chi = np.zeros((3, lat.kvol))
+    with pytest.raises(ChsError):
+        fm.fermion(lat, chi, 1e-8, 0)
+    with pytest.raises(ChsError):
+        fm.fermion(lat, chi, 1e-8, 5)
+    lat.close()

@@ -337,3 +337,24 @@ def test_remez_sets_approximate_powers(name, p, tol):
     rb = remez.evaluate((b, bd), x)
     assert np.max(np.abs(ra / x ** p - 1)) <= tol
     assert np.max(np.abs(rb * x ** p - 1)) <= tol
+
+
+def test_fermion_vs_reference_golden():
+    """orc_fermion against the reference's own fermion routine (flang build,
+    run by tests/golden/make_golden_fermion.py) on the reference's thermalised
+    configuration rhmc/_con: correlators to the solver tolerance, iteration
+    counts within a few per solve."""
+    from chiralheisenberg_amd import checkpoint
+    g = np.load(os.path.join(HERE, "golden", "ref_fermion_l12_ls8.npz"))
+    chi, _, _ = checkpoint.read_con(os.path.join(HERE, "golden", "con_l12_fp64"), kvol=1728)
+    lat = orc.Lattice(12, 8)
+    for k, tol in ((0, 5e-9), (1, 1e-5)):
+        rng = orc.Rng(float(g["rng_seed"]))
+        rng.rano(0)
+        ittt0 = int(12 * rng.rano(0)) + 1
+        c1, c2, it = orc.fermion(lat, chi, float(g["case%d_res" % k]), ittt0, float(g["case%d_hpin" % k]),
+                                 int(g["case%d_imass" % k]))
+        r1, r2 = g["case%d_cferm1" % k], g["case%d_cferm2" % k]
+        assert np.abs(c1 - r1).max() <= tol * np.abs(r1).max()
+        assert np.abs(c2 - r2).max() <= tol * np.abs(r2).max()
+        assert abs(it / 10 - float(g["case%d_aviter" % k])) <= 3
