@@ -15,6 +15,7 @@ Per trajectory, in the reference's order of random-number consumption:
 All reals are FP64 (the reference's -fdefault-real-8 semantics).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -62,10 +63,12 @@ def heatbath_source(lat, rng):
     return R
 
 
-def trajectory(lat, chi, rng, isweep, par=None, sets=None):
+def trajectory(lat, chi, rng, isweep, par=None, sets=None, measure_draw=True):
     """One RHMC trajectory; chi (float64 (3, kvol)) is replaced in place when
-    the proposal is accepted.  Returns a dict of H0, H1, dH, accepted and
-    iteration counts."""
+    the proposal is accepted.  Returns a dict of H0, H1, dH, accepted, the
+    gauge/momentum parts of both Hamiltonians, the mean square velocity and
+    iteration counts.  measure_draw: take the measurement's one yran draw
+    here (run() takes it inside the measurement instead)."""
     p = dict(DEFAULTS)
     if par:
         p.update(par)
@@ -80,8 +83,8 @@ def trajectory(lat, chi, rng, isweep, par=None, sets=None):
     pp[0], pp[1] = ps[0], ps[1]
     ps = rng.gaussp(kv)
     pp[2] = ps[0]
-    (H0, _, _, S0), _, it_h0 = fm.hamilton(lat, Phi, p["rescga"], chit, pp, p["beta"], sets["remez2"], p["am"],
-                                           p["hpin"], p["imass"], isweep)
+    (H0, hg0, hp0, S0), _, it_h0 = fm.hamilton(lat, Phi, p["rescga"], chit, pp, p["beta"], sets["remez2"],
+                                               p["am"], p["hpin"], p["imass"], isweep)
     F, _, _ = fm.force(lat, Phi, p["rescgg"], chit, p["beta"], sets["remez2g"], p["am"], p["hpin"], p["imass"])
     pp -= 0.5 * p["dt"] * F
     proby = 1.0 / p["iterl"]
@@ -95,13 +98,128 @@ def trajectory(lat, chi, rng, isweep, par=None, sets=None):
             pp -= 0.5 * p["dt"] * F
             break
         pp -= p["dt"] * F
-    (H1, _, _, S1), _, it_h1 = fm.hamilton(lat, Phi, p["rescga"], chit, pp, p["beta"], sets["remez2"], p["am"],
-                                           p["hpin"], p["imass"], isweep)
+    (H1, hg1, hp1, S1), _, it_h1 = fm.hamilton(lat, Phi, p["rescga"], chit, pp, p["beta"], sets["remez2"],
+                                               p["am"], p["hpin"], p["imass"], isweep)
     dH = H0 - H1
     accepted = rng.rano(1) < np.exp(dH)
     if accepted:
         chi[...] = chit
-    if p["iprint"] > 0 and isweep % p["iprint"] == 0:
+    if measure_draw and p["iprint"] > 0 and isweep % p["iprint"] == 0:
         rng.rano(0)
-    return dict(H0=H0, H1=H1, S0=S0, S1=S1, dH=dH, accepted=int(accepted), md_steps=steps, it_pf=it_pf,
+    return dict(H0=H0, H1=H1, S0=S0, S1=S1, dH=dH, dS=S0 - S1, accepted=int(accepted), md_steps=steps,
+                hg0=hg0, hp0=hp0, hg1=hg1, hp1=hp1, vel2=float(np.sum(pp * pp) / kv), it_pf=it_pf,
                 it_h0=it_h0, it_h1=it_h1)
+
+
+def read_midout(path):
+    """midout: dt, beta, am3, am, hpin, imass, iterl, iter2 on the first line
+    (rhmc/cheisdwf.f:116)."""
+    v = open(path).readline().split()
+    dt, beta, am3, am, hpin = (float(t) for t in v[:5])
+    imass, iterl, iter2 = (int(t) for t in v[5:8])
+    return dict(dt=dt, beta=beta, am3=am3, am=am, hpin=hpin, imass=imass, iterl=iterl, iter2=iter2)
+
+
+def aux_observables(chi):
+    """cond(a) = <chi_a>, susc(a) = (sum chi_a)^2 / kvol, condmod = |cond|,
+    susctot = sum susc (rhmc/cheisdwf.f:366-384)."""
+    kvol = chi.shape[1]
+    tot = chi.sum(axis=1)
+    cond = tot / kvol
+    susc = tot * tot / kvol
+    return cond, susc, float(np.sqrt(np.sum(cond * cond))), float(susc.sum())
+
+
+def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, iread=1, iwrite=1, iseed=0,
+        rescgm=1e-9, force_precision="fp64", fermion_precision="fp64", sets=None):
+    """The main program dwf3d (rhmc/cheisdwf.f:85-460) with every fermion
+    routine on the GPU.  workdir holds the inputs (midout unless par gives
+    the parameters, con when iread = 1) and receives the reference's outputs:
+    fort.11 (isweep, gauge and momentum action per site), control (dH, dS,
+    accepted), fort.12/13/14 (cond, susc, condmod and susctot every iprint
+    trajectories), fort.5001/5011/5002/5012 (fermion correlators), con (every
+    icheck trajectories and at the end) and output (the averages).  Numbers
+    are written one record per line, space separated.  Returns the averages."""
+    from . import checkpoint
+    p = dict(DEFAULTS)
+    mid = os.path.join(workdir, "midout")
+    if par is None and os.path.exists(mid):
+        par = read_midout(mid)
+    p.update(par or {})
+    p["iprint"] = iprint
+    iter2 = int(iter2 if iter2 is not None else p.get("iter2", 1))
+    icheck = icheck or iprint
+    lat = fm.Lattice(L, Ls, Lt, am3=p.get("am3", 1.0))
+    fm.set_force_precision(lat, force_precision)
+    kv = lat.kvol
+    con = os.path.join(workdir, "con")
+    if iread == 1:
+        chi, seed = checkpoint.sread(con, kv, p["beta"])
+    else:
+        chi, seed = np.full((3, kv), 0.1 / p["beta"]), checkpoint.DEFAULT_SEED
+    if iseed != 0:
+        seed = checkpoint.DEFAULT_SEED
+    rng = RandomStream(seed)
+    rng.rano(0)                      # y = rano(yran, idum) with idum = -1 (rhmc/cheisdwf.f:127-129)
+    sets = sets or {k: rz.load(k) for k in ("remez2", "remez4", "remez2g")}
+    files = {k: open(os.path.join(workdir, k), "a") for k in
+             ("fort.11", "fort.12", "fort.13", "fort.14", "control", "fort.5001", "fort.5011", "fort.5002",
+              "fort.5012")}
+    acc = dict(action=0.0, vel2=0.0, condmod=0.0, susc=0.0, cond=np.zeros(3), ancgm=0.0, yav=0.0, yyav=0.0,
+               naccp=0, ipbp=0, itot=0, it_pf=0, it_h=0)
+    gaction = paction = action = 0.0
+    try:
+        for isweep in range(1, iter2 + 1):
+            t = trajectory(lat, chi, rng, isweep, p, sets, measure_draw=False)
+            if isweep == 1:
+                gaction = t["hg0"] / kv
+            action, paction = t["S0"] / kv, t["hp0"] / kv
+            y = np.exp(t["dH"])
+            acc["yav"] += y
+            acc["yyav"] += y * y
+            if t["accepted"]:
+                acc["naccp"] += 1
+                action, gaction, paction = t["S1"] / kv, t["hg1"] / kv, t["hp1"] / kv
+            files["control"].write("%.17g %.17g %d\n" % (t["dH"], t["dS"], t["accepted"]))
+            files["fort.11"].write("%d %.17g %.17g\n" % (isweep, gaction, paction))
+            acc["action"] += action
+            acc["vel2"] += t["vel2"]
+            acc["itot"] += t["md_steps"]
+            acc["it_pf"] += t["it_pf"]
+            acc["it_h"] += t["it_h0"] + t["it_h1"]
+            if isweep % iprint == 0:
+                cond, susc, condmod, susctot = aux_observables(chi)
+                acc["condmod"] += condmod
+                acc["susc"] += susctot
+                acc["cond"] += cond
+                files["fort.12"].write(" ".join("%.17g" % v for v in cond) + "\n")
+                files["fort.13"].write(" ".join("%.17g" % v for v in susc) + "\n")
+                files["fort.14"].write("%.17g %.17g\n" % (condmod, susctot))
+                ittt0 = int(lat.Lt * rng.rano(0)) + 1
+                c1, c2, its = fm.fermion(lat, chi, rescgm, ittt0, p["hpin"], p["imass"], fermion_precision)
+                for it in range(lat.Lt):
+                    for f, (u1, u2) in enumerate(((5001, 5011), (5002, 5012))):
+                        files["fort.%d" % u1].write("%d %.17g %.17g\n" % (it, c1[f, it].real, c1[f, it].imag))
+                        files["fort.%d" % u2].write("%d %.17g %.17g\n" % (it, c2[f, it].real, c2[f, it].imag))
+                acc["ancgm"] += its / 10.0
+                acc["ipbp"] += 1
+            if isweep % icheck == 0:
+                checkpoint.write_con(con, chi, rng.seed())
+    finally:
+        for f in files.values():
+            f.close()
+    n = iter2
+    ip = max(acc["ipbp"], 1)
+    yav = acc["yav"] / n
+    out = dict(trajectories=n, acceptances=acc["naccp"], av_traj_length=p["dt"] * acc["itot"] / n,
+               exp_mdH=yav, exp_mdH_err=float(np.sqrt(max(acc["yyav"] / n - yav * yav, 0.0) / max(n - 1, 1))),
+               it_pseudofermion=acc["it_pf"] / n, it_acceptance=acc["it_h"] / (2 * n),
+               it_measurement=acc["ancgm"] / ip, condmod=acc["condmod"] / ip, susc=acc["susc"] / ip,
+               cond=(acc["cond"] / ip).tolist(), vel2=acc["vel2"] / n, action=acc["action"] / n)
+    with open(os.path.join(workdir, "output"), "a") as f:
+        for k, v in out.items():
+            f.write("%s %s\n" % (k, v))
+    if iwrite == 1:
+        checkpoint.write_con(con, chi, rng.seed())
+    lat.close()
+    return out

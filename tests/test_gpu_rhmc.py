@@ -74,3 +74,47 @@ def test_trajectories_fp32_force_solves():
     assert [g["accepted"] for g in gpu] == [c["accepted"] for c in cpu]
     assert max(ddh) <= 1e-4
     lat.close()
+
+
+def test_driver_run_matches_oracle(tmp_path):
+    """rhmc.run (program dwf3d: midout + con in, control / fort.11-14 /
+    fort.50xx / con / output out, measurements every iprint trajectories)
+    against the oracle's trajectory loop and fermion measurement driven in
+    the same order of random draws."""
+    from chiralheisenberg_amd import checkpoint
+    L, Ls, iprint, iter2 = 4, 4, 2, 4
+    open(tmp_path / "midout", "w").write("0.3 0.475 1.0 0.00 0.2 1 3 %d\ndt beta am3 am h0 imass iterl iter2\n" % iter2)
+    rs = np.random.RandomState(7)
+    chi0 = rs.randn(3, L ** 3) / np.sqrt(0.475)
+    seed = 122110675517440.0
+    checkpoint.write_con(tmp_path / "con", chi0, seed)
+    out = rhmc.run(str(tmp_path), L, Ls, iprint=iprint, rescgm=1e-10)
+
+    sets = {k: remez.load(k) for k in ("remez2", "remez4", "remez2g")}
+    par = dict(dt=0.3, beta=0.475, hpin=0.2, imass=1, iterl=3, itermax=1000, iprint=0, rescga=1e-9, rescgg=1e-6,
+               am=0.0)
+    olat = orc.Lattice(L, Ls)
+    chi = chi0.copy()
+    orng = orc.Rng(seed)
+    orng.rano(0)
+    control, corr = [], []
+    for isweep in range(1, iter2 + 1):
+        c = orc.rhmc(olat, par, orng, chi, isweep, 1, sets)[0]
+        control.append((c["dH"], c["accepted"]))
+        if isweep % iprint == 0:
+            ittt0 = int(L * orng.rano(0)) + 1
+            corr.append(orc.fermion(olat, chi, 1e-10, ittt0, 0.2, 1))
+    got = [l.split() for l in open(tmp_path / "control")]
+    assert [int(g[2]) for g in got] == [a for _, a in control]
+    for g, (dh, _) in zip(got, control):
+        assert abs(float(g[0]) - dh) <= 1e-6 * max(1.0, abs(dh))   # production tolerances 1e-9 / 1e-6
+    c5001 = np.loadtxt(tmp_path / "fort.5001").reshape(len(corr), L, 3)
+    c5012 = np.loadtxt(tmp_path / "fort.5012").reshape(len(corr), L, 3)
+    for m, (o1, o2, _) in enumerate(corr):
+        assert np.abs(c5001[m, :, 1] + 1j * c5001[m, :, 2] - o1[0]).max() <= 1e-7 * np.abs(o1).max()
+        assert np.abs(c5012[m, :, 1] + 1j * c5012[m, :, 2] - o2[1]).max() <= 1e-7 * np.abs(o2).max()
+    chi_end, seed_end, _ = checkpoint.read_con(tmp_path / "con", kvol=L ** 3)
+    assert np.abs(chi_end - chi).max() <= 1e-9 * np.abs(chi).max()
+    assert seed_end == orng.get()
+    assert out["acceptances"] == sum(a for _, a in control)
+    assert len(open(tmp_path / "fort.14").readlines()) == iter2 // iprint
