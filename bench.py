@@ -78,9 +78,12 @@ def cpu_baseline(L, Ls, beta, fp64_iters, budget_s=20.0):
                       "the FP64 GPU solve" % (n, L, Ls, per_it * 1e3, fp64_iters)}
 
 
-def batch_bench(npf=4, res=1e-9, seed=11):
+BATCH_MIXES = {"full": ((8, 8), (12, 16), (16, 32), (24, 32)), "small": ((8, 8), (12, 8), (12, 16), (16, 16))}
+
+
+def batch_bench(mix="full", npf=4, res=1e-9, seed=11):
     """Batched multi-shift solves (all 12 remez2g shifts x npf pseudofermion
-    fields x lattices L/Ls = 8/8, 12/16, 16/32, 24/32, FP64) in one grouped,
+    fields x the lattices L/Ls of the mix, FP64) in one grouped,
     graph-replayed solve versus the same solves one at a time.  Returns a
     dict with aggregate solves/s and algorithmic HBM GB/s of both."""
     from chiralheisenberg_amd import fermion as fm, batch, remez
@@ -88,7 +91,7 @@ def batch_bench(npf=4, res=1e-9, seed=11):
     n = len(aden)
     rs = np.random.RandomState(seed)
     systems, lats = [], []
-    for L, Ls in ((8, 8), (12, 16), (16, 32), (24, 32)):
+    for L, Ls in BATCH_MIXES[mix]:
         lat = fm.Lattice(L, Ls)
         lats.append(lat)
         chi = fm.DeviceArray.from_numpy(lat, rs.randn(3, lat.kvol) / np.sqrt(0.475))
@@ -125,8 +128,8 @@ def batch_bench(npf=4, res=1e-9, seed=11):
     its = [s.itercg for s in systems]
     same = all(np.array_equal(fm.to_host(s.lat, s.x), r) for s, r in zip(systems, ref)) and its == its1
     B = bytes_moved(its)
-    out = {"workload": "qmrherm x %d: remez2g (%d shifts) x %d pseudofermion fields x L/Ls 8/8, 12/16, 16/32, "
-                       "24/32, FP64, res %g" % (len(systems), n, npf, res),
+    out = {"workload": "qmrherm x %d: remez2g (%d shifts) x %d pseudofermion fields x L/Ls %s, FP64, res %g" % (
+        len(systems), n, npf, ", ".join("%d/%d" % m for m in BATCH_MIXES[mix]), res),
            "solves": len(systems), "itercg": its,
            "one_at_a_time": {"seconds": t_single, "solves_per_s": len(systems) / t_single,
                              "GB_per_s": B / t_single / 1e9},
@@ -135,6 +138,59 @@ def batch_bench(npf=4, res=1e-9, seed=11):
                        "graph_replays": st["graph_replays"]},
            "speedup": t_single / t_batch, "identical_results": bool(same)}
     b.close()
+    return out
+
+
+def learned_bench(fm, lat, chi_d, b_d, tol, npairs=8, epochs=10, lr=2e-3):
+    """The learned preconditioner on the bench system: training pairs from the
+    plain FP32 solver, Adam training on the GPU, then flexible CG (FP32, FP64
+    reliable updates, plain-CG fallback) with and without the network."""
+    from chiralheisenberg_amd import learn
+    t0 = time.perf_counter()
+    src, sol, solver_s = learn.pairs_dev(lat, chi_d, npairs, rng_seed=3, tol=tol)
+    t1 = time.perf_counter()
+    pc = learn.Preconditioner(lat, hidden=16, seed=1)
+    losses = pc.train(src, sol, npairs, epochs=epochs, lr=lr)
+    lat.synchronize()
+    t2 = time.perf_counter()
+    res = {}
+    for use in (True, False):
+        x = fm.spinor_dev(lat, None)
+        x.upload(np.zeros(lat.shape, np.complex128))
+        res[use] = learn.solve_fcg(lat, b_d, x, chi_d, tol, use_nn=use)
+        x.free()
+    src.free()
+    sol.free()
+    nn, pl = res[True], res[False]
+    return {"pairs": npairs, "epochs": epochs, "pair_generation_s": t1 - t0, "training_s": t2 - t1,
+            "loss_first_last": [float(losses[0]), float(losses[-1])],
+            "fcg_with_network": {"iterations": nn["iterations"], "ms": nn["seconds"] * 1e3,
+                                 "fallbacks": nn["restarts"], "true_resid": nn["true_resid"]},
+            "fcg_without_network": {"iterations": pl["iterations"], "ms": pl["seconds"] * 1e3,
+                                    "true_resid": pl["true_resid"]},
+            "iteration_gain": pl["iterations"] / max(nn["iterations"], 1)}
+
+
+def force_bench(fm, lat, chi, beta, res=1e-6, seed=5):
+    """chs_force (remez2g, the reference's force tolerance 1e-6) with FP64 and
+    with FP32-storage multi-shift solves: time and the relative difference of
+    the fermion part."""
+    from chiralheisenberg_amd import remez
+    r2g = remez.load("remez2g")
+    rs = np.random.RandomState(seed)
+    phi = (rs.randn(*lat.shape) + 1j * rs.randn(*lat.shape)) / np.sqrt(2)
+    out = {}
+    F = {}
+    for prec in ("fp64", "fp32"):
+        fm.set_force_precision(lat, prec)
+        fm.force(lat, phi, res, chi, beta, r2g)            # warm-up (allocations)
+        t0 = time.perf_counter()
+        F[prec], _, its = fm.force(lat, phi, res, chi, beta, r2g)
+        out[prec] = {"ms": (time.perf_counter() - t0) * 1e3, "iterations": list(its)}
+    fm.set_force_precision(lat, "fp64")
+    f64, f32 = F["fp64"] - 2 * beta * chi, F["fp32"] - 2 * beta * chi
+    out["fp32_rel_error"] = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
+    out["speedup"] = out["fp64"]["ms"] / out["fp32"]["ms"]
     return out
 
 
@@ -152,6 +208,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-dslash", action="store_true", help="only the roofline kernel and a calibration copy")
     ap.add_argument("--batch", action="store_true", help="only the batched multi-shift workload (one JSON line)")
+    ap.add_argument("--no-extras", action="store_true", help="skip the learned-preconditioner and force figures")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box of this deployment has one MI355X, and the path "
@@ -159,7 +216,7 @@ def main():
         sys.exit(2)
 
     if args.batch:
-        print(json.dumps(batch_bench()), flush=True)
+        print(json.dumps({m: batch_bench(m) for m in BATCH_MIXES}), flush=True)
         return
     from chiralheisenberg_amd import fermion as fm
     lat = fm.Lattice(args.L, args.Ls)
@@ -241,6 +298,11 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": measured_traffic(args.L, args.Ls),
                      "kernel": "k_dsl<F64,F64,false,EPI_STORE>", "bytes_per_launch": 2 * V + C},
     }
+    if not args.no_extras:
+        out["learned"] = learned_bench(fm, lat, chi_d, b_d, args.tol)
+        print("bench: learned %s" % json.dumps(out["learned"]), file=sys.stderr, flush=True)
+        out["force"] = force_bench(fm, lat, chi, args.beta)
+        print("bench: force %s" % json.dumps(out["force"]), file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.L, args.Ls, args.beta, st64["iterations"])
     else:
