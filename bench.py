@@ -38,11 +38,12 @@ def make_inputs(lat, beta, seed=693764523745.0):
     return chi, b
 
 
-def measured_traffic(L, Ls):
-    """HBM bytes per launch of the roofline kernel from the FETCH_SIZE /
-    WRITE_SIZE counter runs recorded in profiles/ (scaled by a plain copy of
-    known size), for the workload they were measured on; None otherwise."""
-    path = os.path.join(ROOT, "profiles", "dslash_traffic.json")
+def measured_traffic(name, L, Ls):
+    """HBM bytes per launch of a kernel from the FETCH_SIZE / WRITE_SIZE
+    counter runs recorded in profiles/<name>_traffic.json (scaled by a plain
+    copy of known size), for the workload they were measured on; None
+    otherwise."""
+    path = os.path.join(ROOT, "profiles", "%s_traffic.json" % name)
     try:
         rec = json.load(open(path))
     except (OSError, ValueError):
@@ -206,7 +207,8 @@ def main():
     ap.add_argument("--form", default="fp32", choices=["fp64", "fp32", "bf16"])
     ap.add_argument("--delta", type=float, default=0.0, help="reliable-update threshold (0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only-dslash", action="store_true", help="only the roofline kernel and a calibration copy")
+    ap.add_argument("--counters", action="store_true",
+                    help="only the roofline kernels and calibration copies (for the counter runs)")
     ap.add_argument("--batch", action="store_true", help="only the batched multi-shift workload (one JSON line)")
     ap.add_argument("--no-extras", action="store_true", help="skip the learned-preconditioner and force figures")
     args = ap.parse_args()
@@ -220,15 +222,19 @@ def main():
         return
     from chiralheisenberg_amd import fermion as fm
     lat = fm.Lattice(args.L, args.Ls)
-    if args.only_dslash:
-        # counter runs: the roofline kernel plus one plain copy of a spinor
-        # (k_convert<F64,F64>: 16 B read + 16 B written per complex number),
-        # which calibrates FETCH_SIZE / WRITE_SIZE against known bytes
-        a = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
-        b2 = fm.spinor_dev(lat, None)
-        fm.check(lat.h, fm.lib().chs_convert_dev(lat.h, b2.ptr, 0, a.ptr, 0, lat.kferm))
-        lat.synchronize()
+    if args.counters:
+        # counter runs: plain copies of an FP64 and an FP32 spinor (k_convert:
+        # 16 / 8 B read and written per complex number) calibrate FETCH_SIZE /
+        # WRITE_SIZE against known bytes; then 20 launches of each roofline
+        # kernel over 4 rotating buffer sets
+        for prec in (0, 1):
+            a = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128), ("fp64", "fp32")[prec])
+            b2 = fm.spinor_dev(lat, None, ("fp64", "fp32")[prec])
+            fm.check(lat.h, fm.lib().chs_convert_dev(lat.h, b2.ptr, prec, a.ptr, prec, lat.kferm))
+            lat.synchronize()
         print("bench: dslash fp64 %.2f us" % (fm.time_dslash(lat, "fp64", False, nbuf=4, iters=20) * 1e3),
+              flush=True)
+        print("bench: cg stencil fp32 %.2f us" % (fm.time_cg_stencil(lat, "fp32", nbuf=4, iters=20) * 1e3),
               flush=True)
         return
     chi, b = make_inputs(lat, args.beta)
@@ -260,16 +266,22 @@ def main():
         args.form, st["seconds"] * 1e3, wall * 1e3, st["iterations"], st["reliable_updates"], st["true_resid"]),
         file=sys.stderr, flush=True)
 
-    # dslash roofline: FP64 dslash, algorithmic bytes 2V + C
+    # rooflines: the FP64 dslash (2 V64 + C), the kernel the metric names, and
+    # the dominant kernel of the timed FP32 solve, its fused CG stencil
+    # (M^dagger M p with the x, r updates and |r|^2; 6 V32 + C).
     kvol = lat.kvol
     V = lat.kferm * 16
+    V32 = lat.kferm * 8
     C = 3 * kvol * 8
+    ms_cg = fm.time_cg_stencil(lat, "fp32", nbuf=4, iters=200)
+    achieved_cg = (6 * V32 + C) / (ms_cg * 1e-3)
     ms = fm.time_dslash(lat, "fp64", False, nbuf=4, iters=200)
     achieved = (2 * V + C) / (ms * 1e-3)
     ms32 = fm.time_dslash(lat, "fp32", False, nbuf=4, iters=200)
     ms16 = fm.time_dslash(lat, "bf16", False, nbuf=4, iters=200)
-    print("bench: dslash fp64 %.2f us (%.0f GB/s), fp32 %.2f us, bf16 %.2f us" % (
-        ms * 1e3, achieved / 1e9, ms32 * 1e3, ms16 * 1e3), file=sys.stderr, flush=True)
+    print("bench: cg stencil fp32 %.2f us (%.0f GB/s); dslash fp64 %.2f us (%.0f GB/s), fp32 %.2f us, bf16 %.2f us" % (
+        ms_cg * 1e3, achieved_cg / 1e9, ms * 1e3, achieved / 1e9, ms32 * 1e3, ms16 * 1e3), file=sys.stderr,
+        flush=True)
 
     out = {
         "metric": METRIC,
@@ -295,8 +307,15 @@ def main():
         "speedup_vs_fp64": st64["seconds"] / max(wall, 1e-12),
         "dslash_us": {"fp64": ms * 1e3, "fp32": ms32 * 1e3, "bf16": ms16 * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(args.L, args.Ls),
-                     "kernel": "k_dsl<F64,F64,false,EPI_STORE>", "bytes_per_launch": 2 * V + C},
+                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic("dslash", args.L, args.Ls),
+                     "kernel": "k_dsl<F64,F64,false,EPI_STORE>", "us": ms * 1e3, "bytes_per_launch": 2 * V + C},
+        "roofline_solve_kernel": {"bound": "hbm", "achieved": achieved_cg / 1e9, "peak": HBM_PEAK / 1e9,
+                                  "unit": "GB/s", "frac": achieved_cg / HBM_PEAK,
+                                  "traffic": measured_traffic("cg_stencil", args.L, args.Ls),
+                                  "kernel": "k_dsl_pair<F32,F32,true,EPI_CG>", "us": ms_cg * 1e3,
+                                  "bytes_per_launch": 6 * V32 + C,
+                                  "note": "isolated, 4 rotating buffer sets (cache-cold); inside the solve its "
+                                          "working set stays in the 256 MB Infinity Cache"},
     }
     if not args.no_extras:
         out["learned"] = learned_bench(fm, lat, chi_d, b_d, args.tol)

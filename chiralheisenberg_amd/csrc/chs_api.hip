@@ -996,6 +996,40 @@ int chs_time_dslash(chs_ctx *c, int prec, int dagger, int nbuf, int iters, doubl
     return CHS_OK;
 }
 
+// The fused CG stencil of the reliable-update solve (M^dagger applied to M p
+// with x += alpha p, r -= alpha M^dagger M p and |r|^2 partials), cycling
+// nbuf sets of its four arrays; alpha = 0, so the traffic is that of a solve.
+int chs_time_cg_stencil(chs_ctx *c, int prec, int nbuf, int iters, double hpin, int imass, double *avg_ms) {
+    if ((prec != CHS_FP64 && prec != CHS_FP32) || nbuf < 1 || iters < 1) return fail(c, CHS_EINVAL, "bad arguments");
+    const size_t bytes = (size_t)c->kferm * prec_bytes(prec);
+    std::vector<void *> a(4 * nbuf, nullptr);
+    double *chi = nullptr;
+    HIPCHK(c, hipMalloc(&chi, 24 * c->kvol));
+    HIPCHK(c, hipMemsetAsync(chi, 0, 24 * c->kvol, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->dscal, 0, sizeof(double), c->stream));   // alpha
+    for (auto &p : a) {
+        HIPCHK(c, hipMalloc(&p, bytes));
+        HIPCHK(c, hipMemsetAsync(p, 0, bytes, c->stream));
+    }
+    auto launch = [&](int i) {
+        void **s = &a[4 * (i % nbuf)];   // M p, p, x, r
+        EpiArgs e = epi(c->partial, c->dscal, s[1], s[2], s[3], nullptr, 0);
+        if (prec == CHS_FP64) dslash_launch<F64, F64, true, EPI_CG>(c, nullptr, s[0], chi, hpin, imass, e);
+        else dslash_launch<F32, F32, true, EPI_CG>(c, nullptr, s[0], chi, hpin, imass, e);
+    };
+    for (int i = 0; i < nbuf; i++) launch(i);   // warm-up
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < iters; i++) launch(i);
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = ms / iters;
+    for (void *p : a) hipFree(p);
+    hipFree(chi);
+    return CHS_OK;
+}
+
 }  // extern "C"
 
 // learned preconditioner and flexible CG (same translation unit)

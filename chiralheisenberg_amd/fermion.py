@@ -301,3 +301,10 @@ def time_dslash(lat, precision="fp64", dagger=False, nbuf=4, iters=100, hpin=0.0
     check(lat.h, lib().chs_time_dslash(lat.h, PRECISIONS[precision], int(bool(dagger)), nbuf, iters, hpin, imass,
                                        ctypes.byref(ms)))
     return ms.value
+
+
+def time_cg_stencil(lat, precision="fp32", nbuf=4, iters=100, hpin=0.0, imass=1):
+    """Average ms per launch of the solve's fused CG stencil (HIP events)."""
+    ms = ctypes.c_double()
+    check(lat.h, lib().chs_time_cg_stencil(lat.h, PRECISIONS[precision], nbuf, iters, hpin, imass, ctypes.byref(ms)))
+    return ms.value

@@ -274,6 +274,13 @@ int chs_qmrherm_batch_dev(chs_batch *b, chs_ms_system *sys, int nsys, const chs_
 int chs_time_dslash(chs_ctx *ctx, int prec, int dagger, int nbuf, int iters,
                     double hpin, int imass, double *avg_ms);
 
+/* Average duration in milliseconds of the fused CG stencil of
+ * chs_solve_mdagm_dev (M^dagger applied to M p, fused with x += alpha p,
+ * r -= alpha M^dagger M p and the |r|^2 partials; prec CHS_FP64 or CHS_FP32),
+ * cycling nbuf sets of its four arrays, HIP events on the context stream.
+ * The dominant kernel of the mixed-precision solve. */
+int chs_time_cg_stencil(chs_ctx *ctx, int prec, int nbuf, int iters, double hpin, int imass, double *avg_ms);
+
 /* ---- random numbers (rhmc/cheisdwf.f:1678-1801) -------------------------- */
 /* Host-side generator state: 48-bit LCG + 97-entry shuffle shared by the
  * yran and yrun pointer streams, FP64 uniforms. */
