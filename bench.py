@@ -82,7 +82,7 @@ def cpu_baseline(L, Ls, beta, fp64_iters, budget_s=20.0):
 BATCH_MIXES = {"full": ((8, 8), (12, 16), (16, 32), (24, 32)), "small": ((8, 8), (12, 8), (12, 16), (16, 16))}
 
 
-def batch_bench(mix="full", npf=4, res=1e-9, seed=11):
+def batch_bench(mix="full", npf=4, res=1e-9, seed=11, precision="fp64"):
     """Batched multi-shift solves (all 12 remez2g shifts x npf pseudofermion
     fields x the lattices L/Ls of the mix, FP64) in one grouped,
     graph-replayed solve versus the same solves one at a time.  Returns a
@@ -102,35 +102,37 @@ def batch_bench(mix="full", npf=4, res=1e-9, seed=11):
 
     def bytes_moved(its):
         # per iteration: two stencils (3V each incl. q / dot), the shift update
-        # reading w, q, q_prev, x and every p_k and writing r, x and every p_k
+        # reading w, q, q_prev and every p_k, writing r and every p_k (V in the
+        # storage precision), and reading and writing the FP64 x
         tot = 0
+        bc = 16 if precision == "fp64" else 8
         for s, it in zip(systems, its):
-            V, C = s.lat.kferm * 16, 3 * s.lat.kvol * 8
-            tot += it * ((12 + 2 * n) * V + 2 * C)
+            V, C, X = s.lat.kferm * bc, 3 * s.lat.kvol * 8, s.lat.kferm * 16
+            tot += it * ((10 + 2 * n) * V + 2 * X + 2 * C)
         return tot
 
     # one at a time (the single-system path, chunked launches)
-    fm.qmrherm_dev(lats[0], systems[0].phi, res, anum, aden, systems[0].chi, systems[0].x)   # warm-up
+    fm.qmrherm_prec_dev(lats[0], systems[0].phi, res, anum, aden, systems[0].chi, systems[0].x, precision)
     lats[0].synchronize()
     t0 = time.perf_counter()
     its1 = []
     for s in systems:
-        its1.append(fm.qmrherm_dev(s.lat, s.phi, res, anum, aden, s.chi, s.x))
+        its1.append(fm.qmrherm_prec_dev(s.lat, s.phi, res, anum, aden, s.chi, s.x, precision))
     for lat in lats:
         lat.synchronize()
     t_single = time.perf_counter() - t0
     ref = [fm.to_host(s.lat, s.x) for s in systems]
     # batched: the first call tunes and captures, the second is timed
     b = batch.Batch()
-    st_first = b.qmrherm(systems)
+    st_first = b.qmrherm(systems, None, precision)
     t0 = time.perf_counter()
-    st = b.qmrherm(systems)
+    st = b.qmrherm(systems, None, precision)
     t_batch = time.perf_counter() - t0
     its = [s.itercg for s in systems]
     same = all(np.array_equal(fm.to_host(s.lat, s.x), r) for s, r in zip(systems, ref)) and its == its1
     B = bytes_moved(its)
-    out = {"workload": "qmrherm x %d: remez2g (%d shifts) x %d pseudofermion fields x L/Ls %s, FP64, res %g" % (
-        len(systems), n, npf, ", ".join("%d/%d" % m for m in BATCH_MIXES[mix]), res),
+    out = {"workload": "qmrherm x %d: remez2g (%d shifts) x %d pseudofermion fields x L/Ls %s, %s storage, res %g" % (
+        len(systems), n, npf, ", ".join("%d/%d" % m for m in BATCH_MIXES[mix]), precision.upper(), res),
            "solves": len(systems), "itercg": its,
            "one_at_a_time": {"seconds": t_single, "solves_per_s": len(systems) / t_single,
                              "GB_per_s": B / t_single / 1e9},
@@ -218,7 +220,8 @@ def main():
         sys.exit(2)
 
     if args.batch:
-        print(json.dumps({m: batch_bench(m) for m in BATCH_MIXES}), flush=True)
+        print(json.dumps({"%s_%s" % (m, p): batch_bench(m, precision=p) for m in BATCH_MIXES
+                          for p in ("fp64", "fp32")}), flush=True)
         return
     from chiralheisenberg_amd import fermion as fm
     lat = fm.Lattice(args.L, args.Ls)

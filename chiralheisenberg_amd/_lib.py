@@ -97,8 +97,9 @@ def _declare(L):
         "chs_batch_create": ([i], v),
         "chs_batch_destroy": ([v], None),
         "chs_batch_last_error": ([v], c),
-        "chs_qmrherm_batch_dev": ([v, ctypes.POINTER(MsSystem), i, ctypes.POINTER(BatchTune),
+        "chs_qmrherm_batch_dev": ([v, ctypes.POINTER(MsSystem), i, i, ctypes.POINTER(BatchTune),
                                    ctypes.POINTER(BatchStats)], i),
+        "chs_qmrherm_prec_dev": ([v, v, d, ip, d, i, v, v, i, i, v, v, v, i], i),
         "chs_rng_set": ([ctypes.POINTER(Rng), d], None),
         "chs_rng_get": ([ctypes.POINTER(Rng)], d),
         "chs_rano": ([ctypes.POINTER(Rng), i], d),

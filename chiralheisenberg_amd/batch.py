@@ -36,9 +36,10 @@ class Batch:
         if rc != _lib.CHS_OK:
             raise ChsError(rc, lib().chs_batch_last_error(self.h).decode())
 
-    def qmrherm(self, systems, tune=None):
-        """Solve every system; sets System.itercg and returns the batch
-        statistics (dict; 'tune' holds the launch parameters used)."""
+    def qmrherm(self, systems, tune=None, precision="fp64"):
+        """Solve every system with its spinors stored in precision ("fp64" or
+        "fp32"); sets System.itercg and returns the batch statistics (dict;
+        'tune' holds the launch parameters used)."""
         arr = (_lib.MsSystem * len(systems))()
         for a, s in zip(arr, systems):
             a.ctx, a.phi, a.chi, a.x = s.lat.h, s.phi.ptr, s.chi.ptr, s.x.ptr
@@ -49,8 +50,8 @@ class Batch:
         if tune is not None:
             t = _lib.BatchTune(int(tune["ept"]), int(tune["chunk"]), int(tune["graph"]), 0.0)
         st = _lib.BatchStats()
-        self._check(lib().chs_qmrherm_batch_dev(self.h, arr, len(systems), ctypes.byref(t) if t else None,
-                                                 ctypes.byref(st)))
+        self._check(lib().chs_qmrherm_batch_dev(self.h, arr, len(systems), _lib.PRECISIONS[precision],
+                                                 ctypes.byref(t) if t else None, ctypes.byref(st)))
         for a, s in zip(arr, systems):
             s.itercg = a.itercg
         tu = st.tune

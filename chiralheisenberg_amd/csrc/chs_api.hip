@@ -526,6 +526,16 @@ int chs_qmrherm_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
     return qmrherm_impl<F64>(c, phi, res, itercg, hpin, imass, anum, aden, ndiag, iflag, chi, x, dsdchi);
 }
 
+int chs_qmrherm_prec_dev(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass,
+                         const double *anum, const double *aden, int ndiag, int iflag, const double *chi, double *x,
+                         double *dsdchi, int prec) {
+    if (prec == CHS_FP32)
+        return qmrherm_impl<F32>(c, phi, res, itercg, hpin, imass, anum, aden, ndiag, iflag, chi, x, dsdchi);
+    if (prec == CHS_FP64)
+        return qmrherm_impl<F64>(c, phi, res, itercg, hpin, imass, anum, aden, ndiag, iflag, chi, x, dsdchi);
+    return fail(c, CHS_EINVAL, "multi-shift precision must be FP64 or FP32");
+}
+
 int chs_set_force_precision(chs_ctx *c, int prec) {
     if (prec != CHS_FP64 && prec != CHS_FP32) return fail(c, CHS_EINVAL, "force precision must be FP64 or FP32");
     c->force_prec = prec;

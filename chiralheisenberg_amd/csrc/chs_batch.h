@@ -4,7 +4,9 @@
 // and a whole step is a handful of launches that a hipGraph replays.
 //
 // A system is one multi-shift solve (rhmc/cheisdwf.f:599-745, iflag = 0) with
-// its own geometry, source, auxiliary field, shifts and device work space.
+// its own geometry, source, auxiliary field, shifts and device work space;
+// the spinors are stored in FP64 or FP32 (S) for the whole batch, the
+// Lanczos scalars, reductions and the rational combination are FP64.
 // Each grouped grid is the concatenation of the systems' own grids in system
 // order; a workgroup finds its system from the table of first blocks and then
 // runs exactly the per-block code of the single-system kernels (dsl_block,
@@ -20,6 +22,8 @@
 #pragma once
 #include "chs_stencil.h"
 
+#include <type_traits>
+
 namespace chs {
 
 struct MsSys {
@@ -29,8 +33,9 @@ struct MsSys {
     double hpin;
     int imass, ndiag;
     int np1, np2;                 // partial counts: stencil grid, vector grid
-    double2 *r, *q0, *q1, *v, *w, *pk;
-    double2 *x;                   // the caller's output: a0 phi + sum_k anum[k+1] x_k
+    void *r, *q0, *q1, *v, *w, *pk;   // spinors in the batch's storage precision
+    double2 *x;                   // the caller's output: a0 phi + sum_k anum[k+1] x_k (FP64)
+    int pairs;                    // FP32 storage on an even L: the paired-site stencil
     const double *anum;           // device copy of the remez numerators
     LzState *lz;
     double *partial, *partial2;
@@ -52,13 +57,15 @@ __device__ __forceinline__ int find_sys(const unsigned *start, int nsys, unsigne
 }
 
 // stage 1: v = M r / beta, q_cur = r / beta; stage 2: w = M^dagger v, <q_cur, w>
-template <bool DAG>
+// (the stencil variant each system's one-at-a-time solve uses)
+template <class S, bool DAG>
 __global__ __launch_bounds__(kBlock) void k_ms_dsl(MsGroup gr, int parity) {
+    typedef typename S::st st;
     const unsigned b = logical_block_x();
     const int j = find_sys(gr.s_dsl, gr.nsys, b);
     const MsSys &s = gr.sys[j];
     if (s.lz->done) return;
-    double2 *qc = parity ? s.q0 : s.q1;
+    st *qc = (st *)(parity ? s.q0 : s.q1);
     EpiArgs ea;
     ea.aux0 = qc;
     ea.aux1 = ea.aux2 = nullptr;
@@ -68,11 +75,25 @@ __global__ __launch_bounds__(kBlock) void k_ms_dsl(MsGroup gr, int parity) {
     if (!DAG) {
         ea.partial = nullptr;
         ea.scal = &s.lz->beta;
-        dsl_block<F64, F64, false, EPI_SCALEQ, 1>(s.lat, s.v, s.r, s.chi, s.hpin, s.imass, ea, lb);
+        if constexpr (std::is_same<S, F32>::value) {
+            if (s.pairs) {
+                dsl_block_pair<S, S, false, EPI_SCALEQ>(s.lat, (st *)s.v, (const st *)s.r, s.chi, s.hpin, s.imass,
+                                                        ea, lb);
+                return;
+            }
+        }
+        dsl_block<S, S, false, EPI_SCALEQ, 1>(s.lat, (st *)s.v, (const st *)s.r, s.chi, s.hpin, s.imass, ea, lb);
     } else {
         ea.partial = s.partial;
         ea.scal = nullptr;
-        dsl_block<F64, F64, true, EPI_DOT, 1>(s.lat, s.w, s.v, s.chi, s.hpin, s.imass, ea, lb);
+        if constexpr (std::is_same<S, F32>::value) {
+            if (s.pairs) {
+                dsl_block_pair<S, S, true, EPI_DOT>(s.lat, (st *)s.w, (const st *)s.v, s.chi, s.hpin, s.imass, ea,
+                                                    lb);
+                return;
+            }
+        }
+        dsl_block<S, S, true, EPI_DOT, 1>(s.lat, (st *)s.w, (const st *)s.v, s.chi, s.hpin, s.imass, ea, lb);
     }
 }
 
@@ -90,26 +111,29 @@ __global__ void k_ms_beta(MsGroup gr) {
 
 // Shift update with EPT element blocks per workgroup (the partials do not
 // depend on EPT); the rational combination accumulates directly into x.
-template <int EPT>
+template <class S, int EPT>
 __device__ __forceinline__ void ms_update(const MsSys &s, int parity, long lb) {
-    const double2 *qc = parity ? s.q0 : s.q1;
-    const double2 *qp = parity ? s.q1 : s.q0;
+    typedef typename S::st st;
+    const st *qc = (const st *)(parity ? s.q0 : s.q1);
+    const st *qp = (const st *)(parity ? s.q1 : s.q0);
+    const st *w = (const st *)s.w;
+    st *r = (st *)s.r, *pk = (st *)s.pk;
     const int iter = s.lz->itercg + 1;
     if (s.ndiag == 25)
-        lz_update<F64, 25, true, EPT>(s.n, s.w, qc, qp, s.r, s.pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
+        lz_update<S, 25, true, EPT>(s.n, w, qc, qp, r, pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
     else if (s.ndiag == 12)
-        lz_update<F64, 12, true, EPT>(s.n, s.w, qc, qp, s.r, s.pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
+        lz_update<S, 12, true, EPT>(s.n, w, qc, qp, r, pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
     else
-        lz_update<F64, 0, true, EPT>(s.n, s.w, qc, qp, s.r, s.pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
+        lz_update<S, 0, true, EPT>(s.n, w, qc, qp, r, pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
 }
 
-template <int EPT>
+template <class S, int EPT>
 __global__ __launch_bounds__(kBlock) void k_ms_update(MsGroup gr, int parity) {
     const unsigned b = logical_block_x();
     const int j = find_sys(gr.s_vec, gr.nsys, b);
     const MsSys &s = gr.sys[j];
     if (s.lz->done) return;
-    ms_update<EPT>(s, parity, (long)(b - gr.s_vec[j]));
+    ms_update<S, EPT>(s, parity, (long)(b - gr.s_vec[j]));
 }
 
 // status[0] = systems done, status[1] = largest itercg (one workgroup)

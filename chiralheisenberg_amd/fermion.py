@@ -285,6 +285,18 @@ def solve_mdagm(lat, b_dev, x_dev, chi_dev, tol=1e-10, precision="fp64", hpin=0.
     return {f: getattr(st, f) for f, _ in _lib.SolveStats._fields_}
 
 
+def qmrherm_prec_dev(lat, phi_dev, res, anum, aden, chi_dev, x_dev, precision="fp64", hpin=0.0, imass=1, iflag=0,
+                     F_dev=None):
+    """qmrherm_dev with the spinors stored in precision ("fp64" or "fp32")."""
+    anum = np.ascontiguousarray(anum, dtype=np.float64)
+    aden = np.ascontiguousarray(aden, dtype=np.float64)
+    it = ctypes.c_int(0)
+    check(lat.h, lib().chs_qmrherm_prec_dev(lat.h, phi_dev.ptr, res, ctypes.byref(it), hpin, imass, _p(anum),
+                                            _p(aden), len(aden), iflag, chi_dev.ptr, x_dev.ptr,
+                                            F_dev.ptr if F_dev is not None else None, PRECISIONS[precision]))
+    return it.value
+
+
 def qmrherm_dev(lat, phi_dev, res, anum, aden, chi_dev, x_dev, hpin=0.0, imass=1, iflag=0, F_dev=None):
     anum = np.ascontiguousarray(anum, dtype=np.float64)
     aden = np.ascontiguousarray(aden, dtype=np.float64)

@@ -105,6 +105,13 @@ int chs_force(chs_ctx *ctx, const double *phi, double res1, double am,
               const double *bden, int ndiag, double *dsdchi, double *x,
               int *its);
 
+/* chs_qmrherm_dev with the spinors stored in prec (CHS_FP64 or CHS_FP32;
+ * Lanczos scalars, reductions, the rational combination x and the derivs
+ * sums stay FP64).  No counterpart in the reference. */
+int chs_qmrherm_prec_dev(chs_ctx *ctx, const double *phi, double res, int *itercg, double hpin, int imass,
+                         const double *anum, const double *aden, int ndiag, int iflag, const double *chi,
+                         double *x, double *dsdchi, int prec);
+
 /* Storage precision of the two multi-shift solves inside chs_force
  * (CHS_FP64, the default, or CHS_FP32: spinors in FP32, Lanczos scalars,
  * reductions and the derivs bilinears and sums in FP64).  No counterpart in
@@ -264,8 +271,11 @@ typedef struct {
 chs_batch *chs_batch_create(int device);   /* NULL on failure: chs_create_error() */
 void chs_batch_destroy(chs_batch *b);
 const char *chs_batch_last_error(chs_batch *b);
-/* tune = NULL: autotune on the first chunks (cached per batch shape) */
-int chs_qmrherm_batch_dev(chs_batch *b, chs_ms_system *sys, int nsys, const chs_batch_tune *tune,
+/* prec: CHS_FP64 or CHS_FP32 storage of every system's spinors (Lanczos
+ * scalars, reductions and x stay FP64; FP32 equals chs_qmrherm_prec_dev with
+ * CHS_FP32 bit for bit).  tune = NULL: autotune on the first chunks (cached
+ * per batch shape). */
+int chs_qmrherm_batch_dev(chs_batch *b, chs_ms_system *sys, int nsys, int prec, const chs_batch_tune *tune,
                           chs_batch_stats *stats);
 
 /* Average duration in milliseconds of `iters` back-to-back dslash launches
