@@ -74,6 +74,8 @@ def _declare(L):
         "chs_derivs": ([v, v, v, d, i, v], i),
         "chs_set_force_precision": ([v, i], i),
         "chs_fermion": ([v, v, d, d, i, i, i, v, v, ip], i),
+        "chs_hmc_force": ([v, v, v, d, d, i, v, d, v, ip], i),
+        "chs_hmc_hamilton": ([v, v, v, d, d, i, v, v, d, v, ip], i),
         "chs_alloc": ([v, sz], v),
         "chs_free": ([v, v], i),
         "chs_upload": ([v, v, v, sz], i),

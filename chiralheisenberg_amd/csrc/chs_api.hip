@@ -772,6 +772,78 @@ int chs_derivs(chs_ctx *c, const double *r, const double *x2, double anum, int i
     return CHS_OK;
 }
 
+// HMC sibling (mh/cheisdwf_hmc.f:393-557): X1 = (M^dagger M)^{-1} Phi by
+// congrad from the guess x0 (hpin = 0: the sibling's operator has no pinning
+// term), dSdchi = 2 (2 beta chi) + derivs(X1, M X1) with a = -1, and the
+// linear chronological guess x0 <- 2 X1 - x0 for the next call.
+int chs_hmc_force(chs_ctx *c, const double *phi, double *x0, double res, double am, int imass, const double *chi,
+                  double beta, double *dsdchi, int *itercg) {
+    (void)am;
+    int rc = ensure_spinors(c, {B_PHI, B_X, B_Q0, B_T2});
+    if (rc) return rc;
+    if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
+    if ((rc = ensure(c, c->b[B_F], 24 * c->kvol))) return rc;
+    const long n = c->kferm;
+    double *dphi = buf<double>(c, B_PHI), *dx1 = buf<double>(c, B_X), *dx0 = buf<double>(c, B_Q0);
+    double *dx2 = buf<double>(c, B_T2), *dchi = buf<double>(c, B_CHI), *dF = buf<double>(c, B_F);
+    if ((rc = h2d_spinor(c, dphi, phi))) return rc;
+    if ((rc = h2d_spinor(c, dx0, x0))) return rc;
+    HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dx1, dx0, 16 * n, hipMemcpyDeviceToDevice, c->stream));
+    int it = 0;
+    rc = chs_congrad_dev(c, dphi, res, &it, 0.0, imass, dchi, dx1);
+    if (rc && rc != CHS_ENOCONV) return rc;   // the sibling, too, carries on at its iteration cap
+    // x0 = 2 X1 - x0 (mh/cheisdwf_hmc.f:434-438)
+    hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, 2.0, (const double2 *)dx1, -1.0,
+                       (double2 *)dx0);
+    dslash_prec<false>(c, dx2, dx1, dchi, 0.0, imass, CHS_FP64);
+    hipLaunchKernelGGL(k_force_init, dim3(nblocks(3 * c->kvol)), dim3(kBlock), 0, c->stream, 3 * c->kvol, 2.0 * beta,
+                       (const double *)dchi, dF);
+    hipLaunchKernelGGL(k_derivs, dim3(nblocks(c->kvol)), dim3(kBlock), 0, c->stream, c->lat, (const double2 *)dx1,
+                       (const double2 *)dx2, -1.0, imass, dF);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(dsdchi, dF, 24 * c->kvol, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = d2h_spinor(c, x0, dx0))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (itercg) *itercg = it;
+    return CHS_OK;
+}
+
+// HMC sibling hamilton (mh/cheisdwf_hmc.f:561-620): hg = 2 beta sum chi^2,
+// hp = sum p^2 / 2, hf = Re Phi^dagger X1 with X1 by congrad from x0, which
+// receives X1.  h4 = (h, hg, hp, s = hg + hf).
+int chs_hmc_hamilton(chs_ctx *c, const double *phi, double *x0, double res, double am, int imass, const double *chi,
+                     const double *pp, double beta, double *h4, int *itercg) {
+    (void)am;
+    int rc = ensure_spinors(c, {B_PHI, B_X});
+    if (rc) return rc;
+    if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
+    if ((rc = ensure(c, c->b[B_PP], 24 * c->kvol))) return rc;
+    double *dphi = buf<double>(c, B_PHI), *dx = buf<double>(c, B_X), *dchi = buf<double>(c, B_CHI);
+    double *dpp = buf<double>(c, B_PP);
+    if ((rc = h2d_spinor(c, dphi, phi))) return rc;
+    if ((rc = h2d_spinor(c, dx, x0))) return rc;
+    HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dpp, pp, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    reduce_launch(c, 3 * c->kvol, dpp, nullptr, 2, c->dscal + 8);
+    reduce_launch(c, 3 * c->kvol, dchi, nullptr, 2, c->dscal + 9);
+    int it = 0;
+    rc = chs_congrad_dev(c, dphi, res, &it, 0.0, imass, dchi, dx);
+    if (rc && rc != CHS_ENOCONV) return rc;
+    reduce_launch(c, c->kferm, dphi, dx, 1, c->dscal + 10);
+    double sums[3];
+    HIPCHK(c, hipMemcpyAsync(sums, c->dscal + 8, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = d2h_spinor(c, x0, dx))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double hp = 0.5 * sums[0], hg = 2.0 * beta * sums[1], hf = sums[2];
+    h4[0] = hg + hp + hf;
+    h4[1] = hg;
+    h4[2] = hp;
+    h4[3] = hg + hf;
+    if (itercg) *itercg = it;
+    return CHS_OK;
+}
+
 int chs_hamilton(chs_ctx *c, const double *phi, double *h, double *hg, double *hp, double *s, double res2,
                  int isweep, double am, double hpin, int imass, const double *chi, const double *pp, double beta,
                  const double *bnum, const double *bden, int ndiag, double *x, int *itercg) {

@@ -161,6 +161,31 @@ def fermion(lat, chi, res, ittt0, hpin=0.0, imass=1, precision="fp64"):
     return c1, c2, it.value
 
 
+def hmc_force(lat, Phi, x0, res, chi, beta, imass=1, am=0.0):
+    """HMC sibling force (mh/cheisdwf_hmc.f:393-557).  x0 (the chronological
+    guess, reference layout) is updated in place to 2 X1 - x0.  Returns
+    (dSdchi, iterations)."""
+    Phi = lat._spinor(Phi)
+    assert x0.dtype == np.complex128 and x0.flags.c_contiguous and x0.shape == lat.shape
+    F = np.empty((3, lat.kvol))
+    it = ctypes.c_int(0)
+    check(lat.h, lib().chs_hmc_force(lat.h, _p(Phi), _p(x0), res, am, imass, _p(lat._field(chi)), beta, _p(F),
+                                     ctypes.byref(it)))
+    return F, it.value
+
+
+def hmc_hamilton(lat, Phi, x0, res, chi, pp, beta, imass=1, am=0.0):
+    """HMC sibling hamilton (mh/cheisdwf_hmc.f:561-620); x0 receives X1.
+    Returns ((h, hg, hp, s), iterations)."""
+    Phi = lat._spinor(Phi)
+    assert x0.dtype == np.complex128 and x0.flags.c_contiguous and x0.shape == lat.shape
+    h4 = np.empty(4)
+    it = ctypes.c_int(0)
+    check(lat.h, lib().chs_hmc_hamilton(lat.h, _p(Phi), _p(x0), res, am, imass, _p(lat._field(chi)),
+                                        _p(lat._field(pp)), beta, _p(h4), ctypes.byref(it)))
+    return tuple(h4), it.value
+
+
 def set_force_precision(lat, precision):
     """Storage precision ("fp64" or "fp32") of the multi-shift solves inside
     force() for this lattice context; hamilton stays FP64."""

@@ -1,0 +1,153 @@
+"""The reference's HMC sibling (program in mh/cheisdwf_hmc.f) with every
+fermion routine on the GPU: two flavours through Phi = M^dagger R, a
+congrad-based force and Hamiltonian sharing one chronological initial guess.
+
+Per trajectory, in the reference's order of random-number consumption
+(mh/cheisdwf_hmc.f:140-330):
+  1. R complex Gaussian (gauss0 per s, Dirac, flavour as in the RHMC), Phi =
+     M^dagger R, the guess X0 = Phi;
+  2. momenta from two gaussp calls;
+  3. H0 = hamilton (X0 <- X1), force with the guess update, leapfrog of random
+     length (one yrun draw per step, end with probability 1/iterl);
+  4. H1 = hamilton; accepted if dH = H0 - H1 >= 0, else if rano(yrun) <= exp(dH).
+The operator is M with hpin = 0 (the sibling has no pinning term); all reals
+FP64.
+"""
+import os
+
+import numpy as np
+
+from . import fermion as fm
+from .rhmc import RandomStream, aux_observables, heatbath_source
+
+DEFAULTS = dict(dt=0.06, beta=0.45, am=0.0, imass=1, iterl=16, itermax=1000, rescga=1e-9, rescgg=1e-6)
+
+
+def trajectory(lat, chi, rng, isweep, par=None):
+    """One HMC trajectory; chi (float64 (3, kvol)) is replaced in place when
+    the proposal is accepted.  Returns a dict of H0, H1, dH, accepted, the
+    Hamiltonian parts, the mean square velocity and iteration counts."""
+    p = dict(DEFAULTS)
+    if par:
+        p.update(par)
+    kv = lat.kvol
+    chit = chi.copy()
+    R = heatbath_source(lat, rng)
+    Phi = fm.dslashd(lat, R, chit, p["am"], 0.0, p["imass"])
+    x0 = Phi.copy()
+    pp = np.empty((3, kv))
+    ps = rng.gaussp(kv)
+    pp[0], pp[1] = ps[0], ps[1]
+    ps = rng.gaussp(kv)
+    pp[2] = ps[0]
+    (H0, hg0, hp0, S0), it_h0 = fm.hmc_hamilton(lat, Phi, x0, p["rescga"], chit, pp, p["beta"], p["imass"])
+    its = 0
+    F, it = fm.hmc_force(lat, Phi, x0, p["rescgg"], chit, p["beta"], p["imass"])
+    its += it
+    pp -= 0.5 * p["dt"] * F
+    proby = 1.0 / p["iterl"]
+    steps = 0
+    for step in range(1, p["itermax"] + 1):
+        chit += p["dt"] * pp
+        F, it = fm.hmc_force(lat, Phi, x0, p["rescgg"], chit, p["beta"], p["imass"])
+        its += it
+        steps = step
+        if rng.rano(1) < proby:
+            pp -= 0.5 * p["dt"] * F
+            break
+        pp -= p["dt"] * F
+    (H1, hg1, hp1, S1), it_h1 = fm.hmc_hamilton(lat, Phi, x0, p["rescga"], chit, pp, p["beta"], p["imass"])
+    dH = H0 - H1
+    accepted = True
+    if dH < 0.0:
+        accepted = not (rng.rano(1) > np.exp(dH))
+    if accepted:
+        chi[...] = chit
+    return dict(H0=H0, H1=H1, S0=S0, S1=S1, dH=dH, dS=S0 - S1, accepted=int(accepted), md_steps=steps,
+                hg0=hg0, hp0=hp0, hg1=hg1, hp1=hp1, vel2=float(np.sum(pp * pp) / kv), it_force=its,
+                it_h0=it_h0, it_h1=it_h1)
+
+
+def read_midout(path):
+    """midout of the sibling: dt, beta, am3, am, imass, iterl, iter2
+    (mh/cheisdwf_hmc.f:78)."""
+    v = open(path).readline().split()
+    dt, beta, am3, am = (float(t) for t in v[:4])
+    imass, iterl, iter2 = (int(t) for t in v[4:7])
+    return dict(dt=dt, beta=beta, am3=am3, am=am, imass=imass, iterl=iterl, iter2=iter2)
+
+
+def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, iread=1, iwrite=1, iseed=0):
+    """The sibling's main program: midout (unless par) and con in; control
+    (dH, dS), fort.11 (isweep, gauge and momentum action per site), fort.12/13/14
+    every iprint trajectories, con every icheck trajectories and at the end,
+    output (averages).  Returns the averages."""
+    from . import checkpoint
+    p = dict(DEFAULTS)
+    mid = os.path.join(workdir, "midout")
+    if par is None and os.path.exists(mid):
+        par = read_midout(mid)
+    p.update(par or {})
+    iter2 = int(iter2 if iter2 is not None else p.get("iter2", 1))
+    lat = fm.Lattice(L, Ls, Lt, am3=p.get("am3", 1.0))
+    kv = lat.kvol
+    con = os.path.join(workdir, "con")
+    if iread == 1:
+        chi, seed = checkpoint.sread(con, kv, p["beta"])
+    else:
+        chi, seed = np.full((3, kv), 0.1 / p["beta"]), checkpoint.DEFAULT_SEED
+    if iseed != 0:
+        seed = 4587658465895.0      # mh/cheisdwf_hmc.f:82
+    rng = RandomStream(seed)
+    rng.rano(0)                      # y = rano(yran, idum) with idum = -1
+    files = {k: open(os.path.join(workdir, k), "a") for k in ("fort.11", "fort.12", "fort.13", "fort.14", "control")}
+    acc = dict(action=0.0, vel2=0.0, condmod=0.0, susc=0.0, cond=np.zeros(3), yav=0.0, yyav=0.0, naccp=0, ipbp=0,
+               itot=0, it_f=0, it_h=0)
+    gaction = paction = action = 0.0
+    try:
+        for isweep in range(1, iter2 + 1):
+            t = trajectory(lat, chi, rng, isweep, p)
+            if isweep == 1:
+                action, gaction, paction = t["S0"] / kv, t["hg0"] / kv, t["hp0"] / kv
+            y = np.exp(t["dH"])
+            acc["yav"] += y
+            acc["yyav"] += y * y
+            files["control"].write("%.17g %.17g\n" % (t["dH"], t["dS"]))
+            if t["accepted"]:
+                acc["naccp"] += 1
+                action, gaction, paction = t["S1"] / kv, t["hg1"] / kv, t["hp1"] / kv
+            files["fort.11"].write("%d %.17g %.17g\n" % (isweep, gaction, paction))
+            acc["action"] += action
+            acc["vel2"] += t["vel2"]
+            acc["itot"] += t["md_steps"]
+            acc["it_f"] += t["it_force"]
+            acc["it_h"] += t["it_h0"] + t["it_h1"]
+            if isweep % iprint == 0:
+                cond, susc, condmod, susctot = aux_observables(chi)
+                acc["condmod"] += condmod
+                acc["susc"] += susctot
+                acc["cond"] += cond
+                acc["ipbp"] += 1
+                files["fort.12"].write(" ".join("%.17g" % v for v in cond) + "\n")
+                files["fort.13"].write(" ".join("%.17g" % v for v in susc) + "\n")
+                files["fort.14"].write("%.17g %.17g\n" % (condmod, susctot))
+            if isweep % icheck == 0:
+                checkpoint.write_con(con, chi, rng.seed())
+    finally:
+        for f in files.values():
+            f.close()
+    n = iter2
+    ip = max(acc["ipbp"], 1)
+    yav = acc["yav"] / n
+    out = dict(trajectories=n, acceptances=acc["naccp"], av_traj_length=p["dt"] * acc["itot"] / n,
+               exp_mdH=yav, exp_mdH_err=float(np.sqrt(max(acc["yyav"] / n - yav * yav, 0.0) / max(n - 1, 1))),
+               it_force=acc["it_f"] / max(acc["itot"], 1), it_acceptance=acc["it_h"] / (2 * n),
+               condmod=acc["condmod"] / ip, susc=acc["susc"] / ip, cond=(acc["cond"] / ip).tolist(),
+               vel2=acc["vel2"] / n, action=acc["action"] / n)
+    with open(os.path.join(workdir, "output"), "a") as f:
+        for k, v in out.items():
+            f.write("%s %s\n" % (k, v))
+    if iwrite == 1:
+        checkpoint.write_con(con, chi, rng.seed())
+    lat.close()
+    return out

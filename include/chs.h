@@ -142,6 +142,19 @@ int chs_hamilton(chs_ctx *ctx, const double *phi, double *h, double *hg,
 int chs_fermion(chs_ctx *ctx, const double *chi, double res, double hpin, int imass, int ittt0, int prec,
                 double *cferm1, double *cferm2, int *iterations);
 
+/* ---- HMC sibling mh/cheisdwf_hmc.f (its operator is M with hpin = 0) ---- */
+/* Replaces force(Phi,X0,dSdchi,iflag,res1,am,imass), mh/cheisdwf_hmc.f:393:
+ * X1 = (M^dagger M)^{-1} Phi by congrad from the guess x0 (/vector/ in the
+ * reference), dSdchi = 4 beta chi - 2 (wall bilinears of X1 and M X1), and
+ * x0 <- 2 X1 - x0.  /trial/ chi and /para/ beta explicit; host arrays. */
+int chs_hmc_force(chs_ctx *ctx, const double *phi, double *x0, double res, double am, int imass, const double *chi,
+                  double beta, double *dsdchi, int *itercg);
+/* Replaces hamilton(Phi,X0,h,hg,hp,s,res2,isweep,am,imass),
+ * mh/cheisdwf_hmc.f:561: h4 = (h, hg = 2 beta sum chi^2, hp = sum p^2 / 2,
+ * s = hg + Re Phi^dagger X1), X1 by congrad from x0, which receives X1. */
+int chs_hmc_hamilton(chs_ctx *ctx, const double *phi, double *x0, double res, double am, int imass,
+                     const double *chi, const double *pp, double beta, double *h4, int *itercg);
+
 /* dSdchi += 2 a wall bilinears of r and x2 = M r.
  * Replaces derivs(R,X2,anum,imass), rhmc/cheisdwf.f:747. */
 int chs_derivs(chs_ctx *ctx, const double *r, const double *x2, double anum,

@@ -494,6 +494,42 @@ int orc_fermion(const orc_lat *g, const double *chi, double res, double hpin,
     return iter;
 }
 
+int orc_hmc_force(const orc_lat *g, const double *phi, double *x0, double res,
+                  const double *chi, double beta, int imass, double *dSdchi) {
+    const long n = kferm_of(g), kvol = kvol_of(g);
+    double *x1 = (double *)malloc(sizeof(double) * 2 * n);
+    double *x2 = (double *)malloc(sizeof(double) * 2 * n);
+    memcpy(x1, x0, sizeof(double) * 2 * n);
+    const int it = orc_congrad(g, phi, res, chi, 0.0, imass, x1);
+    for (long i = 0; i < 2 * n; i++) x0[i] = 2.0 * x1[i] - x0[i];
+    orc_dslash(g, x2, x1, chi, 0.0, imass, 0);
+    for (long i = 0; i < 3 * kvol; i++) dSdchi[i] = 2.0 * beta * chi[i];
+    /* the reference subtracts the bilinears and doubles everything
+     * (mh/cheisdwf_hmc.f:440-552): 2 (2 beta chi) + derivs with a = -1 */
+    for (long i = 0; i < 3 * kvol; i++) dSdchi[i] *= 2.0;
+    orc_derivs(g, dSdchi, x1, x2, -1.0, imass);
+    free(x1); free(x2);
+    return it;
+}
+
+int orc_hmc_hamilton(const orc_lat *g, const double *phi, double *x0, double res,
+                     const double *chi, const double *pp, double beta, int imass,
+                     double *out) {
+    const long n = kferm_of(g), kvol = kvol_of(g);
+    double hp = 0.0, hg = 0.0;
+    for (long i = 0; i < 3 * kvol; i++) hp += pp[i] * pp[i];
+    hp *= 0.5;
+    for (long i = 0; i < 3 * kvol; i++) hg += chi[i] * chi[i];
+    hg *= 2.0 * beta;
+    const int it = orc_congrad(g, phi, res, chi, 0.0, imass, x0);
+    const double hf = orc_redot(n, phi, x0);   /* n complex numbers */
+    out[0] = hg + hp + hf;
+    out[1] = hg;
+    out[2] = hp;
+    out[3] = hg + hf;
+    return it;
+}
+
 int orc_hamilton(const orc_lat *g, const double *phi, double res2,
                  const double *chi, const double *pp, double beta,
                  const double *bnum, const double *bden, int ndiag,

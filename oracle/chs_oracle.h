@@ -85,6 +85,22 @@ int orc_hamilton(const orc_lat *g, const double *phi, double res2,
 int orc_fermion(const orc_lat *g, const double *chi, double res, double hpin,
                 int imass, int ittt0, double *cferm1, double *cferm2);
 
+/* ---- HMC sibling mh/cheisdwf_hmc.f (its operator is M at hpin = 0) --- */
+/* force (mh/cheisdwf_hmc.f:393-557): X1 = (M^dagger M)^{-1} Phi by congrad
+ * from the initial guess x0; dSdchi = 4 beta chi - 2 (wall bilinears of X1,
+ * M X1), i.e. 2 (2 beta chi) plus derivs with a = -1; on exit
+ * x0 = 2 X1 - x0 (the linear chronological guess).  Returns the iterations. */
+int orc_hmc_force(const orc_lat *g, const double *phi, double *x0, double res,
+                  const double *chi, double beta, int imass, double *dSdchi);
+
+/* hamilton (mh/cheisdwf_hmc.f:561-620): hg = 2 beta sum chi^2,
+ * hp = sum p^2 / 2, hf = Re Phi^dagger X1 with X1 = (M^dagger M)^{-1} Phi by
+ * congrad from x0; x0 = X1 on exit; out = h, hg, hp, s = hg + hf.  Returns
+ * the iterations. */
+int orc_hmc_hamilton(const orc_lat *g, const double *phi, double *x0, double res,
+                     const double *chi, const double *pp, double beta, int imass,
+                     double *out);
+
 /* ---- random numbers (rhmc/cheisdwf.f:1678-1801) ---------------------- */
 typedef struct {
     uint64_t s;       /* 48-bit LCG state */

@@ -70,6 +70,10 @@ def lib():
         _LIB.orc_hamilton.restype = i
         _LIB.orc_fermion.argtypes = [L, p, d, d, i, i, p, p]
         _LIB.orc_fermion.restype = i
+        _LIB.orc_hmc_force.argtypes = [L, p, p, d, p, d, i, p]
+        _LIB.orc_hmc_force.restype = i
+        _LIB.orc_hmc_hamilton.argtypes = [L, p, p, d, p, p, d, i, p]
+        _LIB.orc_hmc_hamilton.restype = i
         R = ctypes.POINTER(OrcRng)
         _LIB.orc_rng_set.argtypes = [R, d]
         _LIB.orc_rng_get.argtypes = [R]
@@ -168,6 +172,31 @@ def fermion(lat, chi, res, ittt0, hpin=0.0, imass=1):
     it = lib().orc_fermion(ctypes.byref(lat.c), _ptr(chi), float(res), float(hpin), imass, int(ittt0), _ptr(c1),
                            _ptr(c2))
     return c1, c2, it
+
+
+def hmc_force(lat, phi, x0, res, chi, beta, imass=1):
+    """HMC sibling force (mh/cheisdwf_hmc.f:393-557).  Returns (dSdchi,
+    x0 after the chronological update, iterations)."""
+    phi = _c128(phi, lat)
+    x0 = np.array(x0, dtype=np.complex128, copy=True)
+    chi = np.ascontiguousarray(chi, dtype=np.float64)
+    F = np.zeros((3, lat.kvol))
+    it = lib().orc_hmc_force(ctypes.byref(lat.c), _ptr(phi), _ptr(x0), float(res), _ptr(chi), float(beta), imass,
+                             _ptr(F))
+    return F, x0, it
+
+
+def hmc_hamilton(lat, phi, x0, res, chi, pp, beta, imass=1):
+    """HMC sibling hamilton (mh/cheisdwf_hmc.f:561-620).  Returns ((h, hg, hp,
+    s), x0 after, iterations)."""
+    phi = _c128(phi, lat)
+    x0 = np.array(x0, dtype=np.complex128, copy=True)
+    chi = np.ascontiguousarray(chi, dtype=np.float64)
+    pp = np.ascontiguousarray(pp, dtype=np.float64)
+    out = np.zeros(4)
+    it = lib().orc_hmc_hamilton(ctypes.byref(lat.c), _ptr(phi), _ptr(x0), float(res), _ptr(chi), _ptr(pp),
+                                float(beta), imass, _ptr(out))
+    return tuple(out), x0, it
 
 
 def cg_iterations(lat, phi, chi, niter, hpin=0.0, imass=1):

@@ -1,0 +1,133 @@
+"""The HMC sibling (mh/cheisdwf_hmc.f) on the GPU: force with its
+chronological guess and the Hamiltonian against the reference's own build
+(golden vectors) and the oracle; trajectories against an oracle-driven loop in
+the same order of random draws; the driver's files."""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import phi_field, spinor
+
+pytestmark = pytest.mark.gpu
+
+fm = pytest.importorskip("chiralheisenberg_amd.fermion")
+from chiralheisenberg_amd import checkpoint, hmc  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def golden_inputs():
+    spec = importlib.util.spec_from_file_location("mkh", os.path.join(HERE, "golden", "make_golden_hmc.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    g = np.load(os.path.join(HERE, "golden", "ref_hmc_l12_ls8.npz"))
+    return g, mk.inputs(int(g["seed"]))
+
+
+def test_hmc_force_hamilton_vs_reference_golden():
+    g, (phi, chi, pp, x0, idx) = golden_inputs()
+    lat = fm.Lattice(12, 8)
+    for imass in (1, 3):
+        xf = x0.copy()
+        F, it = fm.hmc_force(lat, phi, xf, float(g["res"]), chi, float(g["beta"]), imass)
+        Fr = g["force_m%d" % imass]
+        assert np.abs(F - Fr).max() <= 1e-9 * np.abs(Fr).max(), np.abs(F - Fr).max()
+        assert np.abs(xf.reshape(-1)[idx] - g["force_x0_m%d_s" % imass]).max() <= 1e-10 * np.abs(xf).max()
+        xh = x0.copy()
+        h, _ = fm.hmc_hamilton(lat, phi, xh, float(g["res"]), chi, pp, float(g["beta"]), imass)
+        assert np.allclose(h, g["ham_m%d" % imass], rtol=1e-12, atol=0), (h, g["ham_m%d" % imass])
+        assert np.abs(xh.reshape(-1)[idx] - g["ham_x0_m%d_s" % imass]).max() <= 1e-10 * np.abs(xh).max()
+    lat.close()
+
+
+def test_hmc_force_vs_oracle_other_size():
+    lat, olat = fm.Lattice(6, 4, 8), orc.Lattice(6, 4, 8)
+    rs = np.random.RandomState(12)
+    phi, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    x0 = 0.1 * spinor(rs, lat.shape)
+    xg = x0.copy()
+    F, it = fm.hmc_force(lat, phi, xg, 1e-11, chi, 0.45, 3)
+    Fo, xo, ito = orc.hmc_force(olat, phi, x0, 1e-11, chi, 0.45, 3)
+    assert np.abs(F - Fo).max() <= 1e-9 * np.abs(Fo).max()
+    assert np.abs(xg - xo).max() <= 1e-9 * np.abs(xo).max()
+    assert abs(it - ito) <= 5
+    lat.close()
+
+
+def oracle_trajectory(olat, chi, rng, par):
+    """The sibling's trajectory on the oracle, same draws as hmc.trajectory."""
+    kv = olat.kvol
+    chit = chi.copy()
+    R = np.empty(olat.shape, np.complex128)
+    Rv = R.reshape(2, 4, kv, olat.Ls)
+    for f in range(2):
+        for d in range(4):
+            for s in range(olat.Ls):
+                ps = rng.gauss0(kv)
+                Rv[f, d, :, s] = ps[0] + 1j * ps[1]
+    Phi = orc.dslash(olat, R, chit, 0.0, par["imass"], True)
+    x0 = Phi.copy()
+    pp = np.empty((3, kv))
+    ps = rng.gaussp(kv)
+    pp[0], pp[1] = ps[0], ps[1]
+    pp[2] = rng.gaussp(kv)[0]
+    (H0, _, _, _), x0, _ = orc.hmc_hamilton(olat, Phi, x0, par["rescga"], chit, pp, par["beta"], par["imass"])
+    F, x0, _ = orc.hmc_force(olat, Phi, x0, par["rescgg"], chit, par["beta"], par["imass"])
+    pp -= 0.5 * par["dt"] * F
+    steps = 0
+    for step in range(1, par["itermax"] + 1):
+        chit += par["dt"] * pp
+        F, x0, _ = orc.hmc_force(olat, Phi, x0, par["rescgg"], chit, par["beta"], par["imass"])
+        steps = step
+        if rng.rano(1) < 1.0 / par["iterl"]:
+            pp -= 0.5 * par["dt"] * F
+            break
+        pp -= par["dt"] * F
+    (H1, _, _, _), x0, _ = orc.hmc_hamilton(olat, Phi, x0, par["rescga"], chit, pp, par["beta"], par["imass"])
+    dH = H0 - H1
+    acc = True
+    if dH < 0:
+        acc = not (rng.rano(1) > np.exp(dH))
+    if acc:
+        chi[...] = chit
+    return dict(dH=dH, accepted=int(acc), md_steps=steps)
+
+
+@pytest.mark.parametrize("imass", [1, 3])
+def test_hmc_trajectories_match_oracle(imass):
+    L, Ls, ntraj = 4, 4, 6
+    par = dict(dt=0.06, beta=0.45, am=0.0, imass=imass, iterl=3, itermax=1000, rescga=1e-11, rescgg=1e-10)
+    rs = np.random.RandomState(40 + imass)
+    chi0 = rs.randn(3, L ** 3) / np.sqrt(0.45)
+    seed = 693764523745.0
+    lat = fm.Lattice(L, Ls)
+    chi_g = chi0.copy()
+    rng = hmc.RandomStream(seed)
+    gpu = [hmc.trajectory(lat, chi_g, rng, i + 1, par) for i in range(ntraj)]
+    olat = orc.Lattice(L, Ls)
+    chi_c = chi0.copy()
+    orng = orc.Rng(seed)
+    cpu = [oracle_trajectory(olat, chi_c, orng, par) for _ in range(ntraj)]
+    print("accepted", [g["accepted"] for g in gpu], "dH", [round(c["dH"], 4) for c in cpu])
+    assert [g["md_steps"] for g in gpu] == [c["md_steps"] for c in cpu]
+    assert [g["accepted"] for g in gpu] == [c["accepted"] for c in cpu]
+    for g, c in zip(gpu, cpu):
+        assert abs(g["dH"] - c["dH"]) <= 1e-7 * max(1.0, abs(c["dH"]))   # H ~ 1e3, solves at 1e-11
+    assert np.abs(chi_g - chi_c).max() <= 1e-9 * np.abs(chi_c).max()
+    assert rng.seed() == orng.get()
+    lat.close()
+
+
+def test_hmc_run_files(tmp_path):
+    open(tmp_path / "midout", "w").write("0.3 0.45 1.0 0.00 1 3 3\ndt beta am3 am imass iterl iter2\n")
+    rs = np.random.RandomState(3)
+    checkpoint.write_con(tmp_path / "con", rs.randn(3, 64) / np.sqrt(0.45), 693764523745.0)
+    out = hmc.run(str(tmp_path), 4, 4, iprint=1)
+    assert out["trajectories"] == 3
+    assert len(open(tmp_path / "control").readlines()) == 3
+    assert len(open(tmp_path / "fort.14").readlines()) == 3
+    chi, seed, _ = checkpoint.read_con(tmp_path / "con", kvol=64)
+    assert np.all(np.isfinite(chi)) and seed != 693764523745.0

@@ -358,3 +358,23 @@ def test_fermion_vs_reference_golden():
         assert np.abs(c1 - r1).max() <= tol * np.abs(r1).max()
         assert np.abs(c2 - r2).max() <= tol * np.abs(r2).max()
         assert abs(it / 10 - float(g["case%d_aviter" % k])) <= 3
+
+
+def test_hmc_sibling_vs_reference_golden():
+    """orc_hmc_force / orc_hmc_hamilton against the reference's HMC sibling
+    (mh/cheisdwf_hmc.f, flang build, tests/golden/make_golden_hmc.py)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mkh", os.path.join(HERE, "golden", "make_golden_hmc.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    g = np.load(os.path.join(HERE, "golden", "ref_hmc_l12_ls8.npz"))
+    phi, chi, pp, x0, idx = mk.inputs(int(g["seed"]))
+    lat = orc.Lattice(12, 8)
+    for imass in (1, 3):
+        F, x0f, _ = orc.hmc_force(lat, phi, x0, float(g["res"]), chi, float(g["beta"]), imass)
+        Fr = g["force_m%d" % imass]
+        assert np.abs(F - Fr).max() <= 1e-9 * np.abs(Fr).max()
+        assert np.abs(x0f.reshape(-1)[idx] - g["force_x0_m%d_s" % imass]).max() <= 1e-10 * np.abs(x0f).max()
+        h, x0h, _ = orc.hmc_hamilton(lat, phi, x0, float(g["res"]), chi, pp, float(g["beta"]), imass)
+        assert np.allclose(h, g["ham_m%d" % imass], rtol=1e-12, atol=0)
+        assert np.abs(x0h.reshape(-1)[idx] - g["ham_x0_m%d_s" % imass]).max() <= 1e-10 * np.abs(x0h).max()
