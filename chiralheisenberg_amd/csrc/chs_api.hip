@@ -952,12 +952,14 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
                                                      epi(c->partial, nullptr, nullptr, nullptr, nullptr, skip));
                 hipLaunchKernelGGL(k_rcg_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->cg,
                                    (const double *)c->partial, np);
+                // r -= alpha M^dagger M p (the x increment moves to the next kernel)
                 dslash_launch<S, SA, true, EPI_CG>(c, nullptr, mp, chi, hpin, imass,
-                                                  epi(c->partial, &c->cg->alpha, ps, xs, rs, skip, 0));
+                                                  epi(c->partial, &c->cg->alpha, ps, xs, rs, skip, 1));
                 hipLaunchKernelGGL(k_rcg_beta, dim3(1), dim3(kBlock), 0, c->stream, c->cg,
                                    (const double *)c->partial, np);
-                hipLaunchKernelGGL((k_cg_pupdate<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
-                                   (typename S::st *)ps, (const typename SA::st *)rs, (const CgState *)c->cg);
+                hipLaunchKernelGGL((k_cg_xpupdate<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
+                                   (typename S::st *)ps, (const typename SA::st *)rs, (typename SA::st *)xs,
+                                   (const CgState *)c->cg);
             }
         };
         GraphGuard gg;
@@ -977,7 +979,10 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
             HIPCHK(c, hipMemcpyAsync(&h, c->cg, sizeof h, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             if (!h.hold) continue;
+            // the increment of the step that raised the hold, then the
             // reliable update: x += xs; r64 = b - M^dagger M x; rs = r64
+            hipLaunchKernelGGL((k_cg_xupdate<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
+                               (const typename S::st *)ps, (typename SA::st *)xs, (const CgState *)c->cg);
             hipLaunchKernelGGL((k_accum_x<SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, x,
                                (typename SA::st *)xs);
             mdagm64(c, t2, x, t1, chi, hpin, imass);
@@ -1079,8 +1084,8 @@ int chs_time_dslash(chs_ctx *c, int prec, int dagger, int nbuf, int iters, doubl
 }
 
 // The fused CG stencil of the reliable-update solve (M^dagger applied to M p
-// with x += alpha p, r -= alpha M^dagger M p and |r|^2 partials), cycling
-// nbuf sets of its four arrays; alpha = 0, so the traffic is that of a solve.
+// with r -= alpha M^dagger M p and |r|^2 partials), cycling nbuf sets of its
+// arrays; alpha = 0, so the traffic is that of a solve.
 int chs_time_cg_stencil(chs_ctx *c, int prec, int nbuf, int iters, double hpin, int imass, double *avg_ms) {
     if ((prec != CHS_FP64 && prec != CHS_FP32) || nbuf < 1 || iters < 1) return fail(c, CHS_EINVAL, "bad arguments");
     const size_t bytes = (size_t)c->kferm * prec_bytes(prec);
@@ -1095,7 +1100,7 @@ int chs_time_cg_stencil(chs_ctx *c, int prec, int nbuf, int iters, double hpin, 
     }
     auto launch = [&](int i) {
         void **s = &a[4 * (i % nbuf)];   // M p, p, x, r
-        EpiArgs e = epi(c->partial, c->dscal, s[1], s[2], s[3], nullptr, 0);
+        EpiArgs e = epi(c->partial, c->dscal, s[1], s[2], s[3], nullptr, 1);
         if (prec == CHS_FP64) dslash_launch<F64, F64, true, EPI_CG>(c, nullptr, s[0], chi, hpin, imass, e);
         else dslash_launch<F32, F32, true, EPI_CG>(c, nullptr, s[0], chi, hpin, imass, e);
     };

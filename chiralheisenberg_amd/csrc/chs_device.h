@@ -596,6 +596,46 @@ __global__ __launch_bounds__(kBlock) void k_cg_pupdate(long n, typename S::st *_
     }
 }
 
+// x += alpha p and p = r + beta p in one pass: the x increment of the step
+// that produced r and the next direction both use the old p (so the fused CG
+// stencil no longer reads p and x: one spinor fewer per iteration)
+template <class S, class SA>
+__global__ __launch_bounds__(kBlock) void k_cg_xpupdate(long n, typename S::st *__restrict__ p,
+                                                        const typename SA::st *__restrict__ r,
+                                                        typename SA::st *__restrict__ x,
+                                                        const CgState *__restrict__ st) {
+    if (st->halt) return;
+    const long i = logical_block() * kBlock + threadIdx.x;
+    typedef typename S::rt T;
+    typedef typename SA::rt TA;
+    if (i < n) {
+        const T b = (T)st->beta;
+        const TA a = (TA)st->alpha;
+        const cx<TA> rv = SA::ld(r, i);
+        const cx<T> pv = S::ld(p, i);
+        cx<TA> xv = SA::ld(x, i);
+        xv = {xv.r + a * (TA)pv.r, xv.i + a * (TA)pv.i};
+        SA::sto(x, i, xv);
+        S::sto(p, i, cx<T>{(T)rv.r + b * pv.r, (T)rv.i + b * pv.i});
+    }
+}
+
+// x += alpha p alone: the last step's increment when the solve stopped for
+// the host (the fused update above did not run)
+template <class S, class SA>
+__global__ __launch_bounds__(kBlock) void k_cg_xupdate(long n, const typename S::st *__restrict__ p,
+                                                       typename SA::st *__restrict__ x,
+                                                       const CgState *__restrict__ st) {
+    const long i = logical_block() * kBlock + threadIdx.x;
+    typedef typename SA::rt TA;
+    if (i < n) {
+        const TA a = (TA)st->alpha;
+        const cx<typename S::rt> pv = S::ld(p, i);
+        cx<TA> xv = SA::ld(x, i);
+        SA::sto(x, i, cx<TA>{xv.r + a * (TA)pv.r, xv.i + a * (TA)pv.i});
+    }
+}
+
 // reliable-update CG: alpha = rnorm2 / |M p|^2
 __device__ __forceinline__ void rcg_alpha_step(CgState *st, double pap) { st->alpha = st->rnorm2 / pap; }
 

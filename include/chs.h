@@ -298,10 +298,10 @@ int chs_time_dslash(chs_ctx *ctx, int prec, int dagger, int nbuf, int iters,
                     double hpin, int imass, double *avg_ms);
 
 /* Average duration in milliseconds of the fused CG stencil of
- * chs_solve_mdagm_dev (M^dagger applied to M p, fused with x += alpha p,
+ * chs_solve_mdagm_dev (M^dagger applied to M p, fused with
  * r -= alpha M^dagger M p and the |r|^2 partials; prec CHS_FP64 or CHS_FP32),
- * cycling nbuf sets of its four arrays, HIP events on the context stream.
- * The dominant kernel of the mixed-precision solve. */
+ * cycling nbuf sets of its arrays, HIP events on the context stream.  The
+ * dominant kernel of the mixed-precision solve. */
 int chs_time_cg_stencil(chs_ctx *ctx, int prec, int nbuf, int iters, double hpin, int imass, double *avg_ms);
 
 /* ---- random numbers (rhmc/cheisdwf.f:1678-1801) -------------------------- */
