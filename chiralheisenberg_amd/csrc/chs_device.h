@@ -636,34 +636,33 @@ __global__ __launch_bounds__(kBlock) void k_cg_xupdate(long n, const typename S:
     }
 }
 
-// reliable-update CG: alpha = rnorm2 / |M p|^2
-__device__ __forceinline__ void rcg_alpha_step(CgState *st, double pap) { st->alpha = st->rnorm2 / pap; }
+// Reliable-update CG scalars.  alpha = rnorm2 / |M p|^2; after |r_s|^2 of
+// the iterated residual either a plain step (beta) or a hold for the host
+// (reliable update due, converged candidate, iteration cap, or a non-finite
+// value).  The kernels read the state together with the partials (one
+// memory round trip instead of three: halt flag, partials, then the fields)
+// and test halt only before storing.
+__global__ void k_rcg_alpha(CgState *st, const double *__restrict__ partial, int np) {
+    const int halt = st->halt;
+    const double rn = st->rnorm2;
+    const double t = sum_partials(partial, np);
+    if (threadIdx.x == 0 && !halt) st->alpha = rn / t;
+}
 
-// reliable-update CG after |r_s|^2 of the iterated residual: a plain step
-// (beta), or a hold for the host (reliable update due, converged candidate,
-// iteration cap, or a non-finite value).
-__device__ __forceinline__ void rcg_beta_step(CgState *st, double rn) {
-    st->iter += 1;
-    if (rn < st->delta2 * st->maxr2 || rn < st->stop2 || st->iter >= st->maxiter || !(rn == rn)) {
+__global__ void k_rcg_beta(CgState *st, const double *__restrict__ partial, int np) {
+    const int halt = st->halt, iter = st->iter + 1, maxiter = st->maxiter;
+    const double rn_old = st->rnorm2, d2 = st->delta2, maxr2 = st->maxr2, stop2 = st->stop2;
+    const double rn = sum_partials(partial, np);
+    if (threadIdx.x != 0 || halt) return;
+    st->iter = iter;
+    if (rn < d2 * maxr2 || rn < stop2 || iter >= maxiter || !(rn == rn)) {
         st->hold = 1;
         st->halt = 1;
         st->rn_hold = rn;
     } else {
-        st->beta = rn / st->rnorm2;
+        st->beta = rn / rn_old;
         st->rnorm2 = rn;
     }
-}
-
-__global__ void k_rcg_alpha(CgState *st, const double *__restrict__ partial, int np) {
-    if (st->halt) return;
-    const double t = sum_partials(partial, np);
-    if (threadIdx.x == 0) rcg_alpha_step(st, t);
-}
-
-__global__ void k_rcg_beta(CgState *st, const double *__restrict__ partial, int np) {
-    if (st->halt) return;
-    const double rn = sum_partials(partial, np);
-    if (threadIdx.x == 0) rcg_beta_step(st, rn);
 }
 
 // x64 += xs (low precision accumulator), xs = 0
