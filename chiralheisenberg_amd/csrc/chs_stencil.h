@@ -165,31 +165,40 @@ __device__ __forceinline__ void stencil(const Lat &lat, const typename S::st *__
     constexpr int dp = DAG ? 2 : 0;
     const bool wall_ok = (imass == 1 || imass == 3);
     const bool last = (s + 1 == Ls), first = (s == 0);
-    if (!last) {
+    // The s-neighbour loads are unconditional (clamped to the centre at the
+    // walls, where the value is then discarded by a select): inside a branch
+    // each would be a memory round trip of its own after the x, y, t loads.
+    // Subtracting the selected +0 leaves o bit-identical.
+    {
+        const unsigned gsp = last ? g : g + ss, gsm = first ? g : g - ss;
+        const cx<T> zero = {T(0), T(0)};
 #pragma unroll
-        for (int ff = 0; ff < NF; ff++) {
-            o[ff][dn] = csub(o[ff][dn], S::ld(pl[ff][dn], g + ss));
-            o[ff][dn + 1] = csub(o[ff][dn + 1], S::ld(pl[ff][dn + 1], g + ss));
-        }
-    }
-    if (!first) {
+        for (int ff = 0; ff < NF; ff++)
 #pragma unroll
-        for (int ff = 0; ff < NF; ff++) {
-            o[ff][dp] = csub(o[ff][dp], S::ld(pl[ff][dp], g - ss));
-            o[ff][dp + 1] = csub(o[ff][dp + 1], S::ld(pl[ff][dp + 1], g - ss));
-        }
+            for (int dd = 0; dd < 2; dd++) {
+                const cx<T> up = S::ld(pl[ff][dn + dd], gsp), dw = S::ld(pl[ff][dp + dd], gsm);
+                o[ff][dn + dd] = csub(o[ff][dn + dd], last ? zero : up);
+                o[ff][dp + dd] = csub(o[ff][dp + dd], first ? zero : dw);
+            }
     }
     if ((first || last) && wall_ok) {
+        const st *q0 = in, *q1 = in + 4 * P;   // flavour 0 / 1 planes at d = 0
+        const unsigned src = last ? elem<LAYOUT>(0, site, Ls, kvol) : elem<LAYOUT>(Ls - 1, site, Ls, kvol);
+        // all wall operands are loaded before any is used (one round trip)
+        cx<T> v[2][2];
+#pragma unroll
+        for (int dd = 0; dd < 2; dd++) {
+            const int d = last ? dn + dd : dp + dd;
+            v[dd][0] = S::ld(q0 + P * (size_t)d, src);
+            v[dd][1] = S::ld(q1 + P * (size_t)d, src);
+        }
         const T c1 = (T)chi[site], c2 = (T)chi[site + kvol];
         double c3d = chi[site + 2u * kvol];
         if (imass == 1 && site == 0) c3d += hpin;
         const T c3 = (T)c3d;
-        const st *q0 = in, *q1 = in + 4 * P;   // flavour 0 / 1 planes at d = 0
-        const unsigned src = last ? elem<LAYOUT>(0, site, Ls, kvol) : elem<LAYOUT>(Ls - 1, site, Ls, kvol);
 #pragma unroll
         for (int dd = 0; dd < 2; dd++) {
-            const int d = last ? dn + dd : dp + dd;
-            const cx<T> v0 = S::ld(q0 + P * (size_t)d, src), v1 = S::ld(q1 + P * (size_t)d, src);
+            const cx<T> v0 = v[dd][0], v1 = v[dd][1];
             cx<T> w0 = {0, 0}, w1 = {0, 0};
             wall_apply<T>(w0, w1, v0, v1, c1, c2, c3, imass, last);
 #pragma unroll
@@ -234,6 +243,15 @@ __device__ __forceinline__ void dsl_block(const Lat &lat, typename S::st *__rest
             site = g / Ls;
             s = g - site * Ls;
         }
+        // the dot product's operand is loaded before the stencil, so that its
+        // latency overlaps the neighbour loads instead of following them (the
+        // CG update's r is not: read early, it slowed the kernel after it)
+        cx<T> apre[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const size_t po = P * (size_t)(d + 4 * fb);
+            if (EPI == EPI_DOT) apre[d] = S::ld((const typename S::st *)ea.aux0 + po, g);
+        }
         cx<T> o[1][4], c[1][4];
         stencil<S, DAG, 1, LAYOUT>(lat, in, chi, hpin, imass, s, site, fb, o, c);
         T acc = 0;
@@ -262,10 +280,7 @@ __device__ __forceinline__ void dsl_block(const Lat &lat, typename S::st *__rest
             } else {
                 store_nt<S>(out + po, g, o[0][d]);
                 if (EPI == EPI_NORM) acc += o[0][d].r * o[0][d].r + o[0][d].i * o[0][d].i;
-                if (EPI == EPI_DOT) {
-                    const cx<T> av = S::ld((const typename S::st *)ea.aux0 + po, g);
-                    acc += av.r * o[0][d].r + av.i * o[0][d].i;
-                }
+                if (EPI == EPI_DOT) acc += apre[d].r * o[0][d].r + apre[d].i * o[0][d].i;
             }
         }
         part = (double)acc;
@@ -355,25 +370,32 @@ __device__ __forceinline__ void stencil_pair(const Lat &lat, const typename S::s
     constexpr int dp = DAG ? 2 : 0;
     const bool wall_ok = (imass == 1 || imass == 3);
     const bool last = (s + 1 == Ls), first = (s == 0);
-    if (!last) {
+    // unconditional s-neighbour loads, clamped at the walls (as in stencil())
+    {
+        const unsigned gsp = last ? g : g + kvol, gsm = first ? g : g - kvol;
+        const cx<T> zero = {T(0), T(0)};
 #pragma unroll
         for (int dd = 0; dd < 2; dd++) {
-            cx<T> a, b;
-            S::ld2(pl[dn + dd], g + kvol, a, b);
-            o0[dn + dd] = csub(o0[dn + dd], a);
-            o1[dn + dd] = csub(o1[dn + dd], b);
-        }
-    }
-    if (!first) {
-#pragma unroll
-        for (int dd = 0; dd < 2; dd++) {
-            cx<T> a, b;
-            S::ld2(pl[dp + dd], g - kvol, a, b);
-            o0[dp + dd] = csub(o0[dp + dd], a);
-            o1[dp + dd] = csub(o1[dp + dd], b);
+            cx<T> a, b, e, f;
+            S::ld2(pl[dn + dd], gsp, a, b);
+            S::ld2(pl[dp + dd], gsm, e, f);
+            o0[dn + dd] = csub(o0[dn + dd], last ? zero : a);
+            o1[dn + dd] = csub(o1[dn + dd], last ? zero : b);
+            o0[dp + dd] = csub(o0[dp + dd], first ? zero : e);
+            o1[dp + dd] = csub(o1[dp + dd], first ? zero : f);
         }
     }
     if ((first || last) && wall_ok) {
+        const st *q0 = in, *q1 = in + 4 * P;   // flavour 0 / 1 planes at d = 0
+        const unsigned src = last ? site : (Ls - 1) * kvol + site;
+        // all wall operands are loaded before any is used (one round trip)
+        cx<T> v[2][4];
+#pragma unroll
+        for (int dd = 0; dd < 2; dd++) {
+            const int d = last ? dn + dd : dp + dd;
+            S::ld2(q0 + P * (size_t)d, src, v[dd][0], v[dd][1]);
+            S::ld2(q1 + P * (size_t)d, src, v[dd][2], v[dd][3]);
+        }
         const double2 ch1 = *reinterpret_cast<const double2 *>(chi + site);
         const double2 ch2 = *reinterpret_cast<const double2 *>(chi + site + kvol);
         const double2 ch3 = *reinterpret_cast<const double2 *>(chi + site + 2u * kvol);
@@ -381,14 +403,10 @@ __device__ __forceinline__ void stencil_pair(const Lat &lat, const typename S::s
         double c3ad = ch3.x;
         if (imass == 1 && site == 0) c3ad += hpin;
         const T c3a = (T)c3ad, c3b = (T)ch3.y;
-        const st *q0 = in, *q1 = in + 4 * P;   // flavour 0 / 1 planes at d = 0
-        const unsigned src = last ? site : (Ls - 1) * kvol + site;
 #pragma unroll
         for (int dd = 0; dd < 2; dd++) {
             const int d = last ? dn + dd : dp + dd;
-            cx<T> v0a, v0b, v1a, v1b;
-            S::ld2(q0 + P * (size_t)d, src, v0a, v0b);
-            S::ld2(q1 + P * (size_t)d, src, v1a, v1b);
+            const cx<T> v0a = v[dd][0], v0b = v[dd][1], v1a = v[dd][2], v1b = v[dd][3];
             cx<T> w0a = {0, 0}, w1a = {0, 0}, w0b = {0, 0}, w1b = {0, 0};
             wall_apply<T>(w0a, w1a, v0a, v1a, c1a, c2a, c3a, imass, last);
             wall_apply<T>(w0b, w1b, v0b, v1b, c1b, c2b, c3b, imass, last);
@@ -419,6 +437,13 @@ __device__ __forceinline__ void dsl_block_pair(const Lat &lat, typename S::st *_
         const unsigned g = 2 * g2;
         const unsigned s = g / kvol;
         const unsigned site = g - s * kvol;
+        // epilogue operands loaded ahead of the stencil (see dsl_block)
+        cx<T> apa[4], apb[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const size_t po = P * (size_t)(d + 4 * fb);
+            if (EPI == EPI_DOT) S::ld2((const typename S::st *)ea.aux0 + po, g, apa[d], apb[d]);
+        }
         cx<T> o0[4], o1[4], c0[4], c1[4];
         stencil_pair<S, DAG>(lat, in, chi, hpin, imass, s, site, fb, o0, o1, c0, c1);
         T acc = 0;
@@ -453,11 +478,8 @@ __device__ __forceinline__ void dsl_block_pair(const Lat &lat, typename S::st *_
                 store_nt2<S>(out + po, g, S::pack2(o0[d], o1[d]));
                 if (EPI == EPI_NORM)
                     acc += o0[d].r * o0[d].r + o0[d].i * o0[d].i + o1[d].r * o1[d].r + o1[d].i * o1[d].i;
-                if (EPI == EPI_DOT) {
-                    cx<T> a, b;
-                    S::ld2((const typename S::st *)ea.aux0 + po, g, a, b);
-                    acc += a.r * o0[d].r + a.i * o0[d].i + b.r * o1[d].r + b.i * o1[d].i;
-                }
+                if (EPI == EPI_DOT)
+                    acc += apa[d].r * o0[d].r + apa[d].i * o0[d].i + apb[d].r * o1[d].r + apb[d].i * o1[d].i;
             }
         }
         part = (double)acc;

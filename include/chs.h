@@ -293,15 +293,17 @@ int chs_qmrherm_batch_dev(chs_batch *b, chs_ms_system *sys, int nsys, int prec, 
 
 /* Average duration in milliseconds of `iters` back-to-back dslash launches
  * cycling through nbuf (in, out) device buffer pairs, measured with HIP
- * events on the context stream.  Used for the HBM roofline figure. */
+ * events on the context stream.  Used for the HBM roofline figure.  Inputs
+ * and phi hold hashed uniform random values: all-zero buffers would time
+ * ~15% fast on this hardware. */
 int chs_time_dslash(chs_ctx *ctx, int prec, int dagger, int nbuf, int iters,
                     double hpin, int imass, double *avg_ms);
 
 /* Average duration in milliseconds of the fused CG stencil of
  * chs_solve_mdagm_dev (M^dagger applied to M p, fused with
  * r -= alpha M^dagger M p and the |r|^2 partials; prec CHS_FP64 or CHS_FP32),
- * cycling nbuf sets of its arrays, HIP events on the context stream.  The
- * dominant kernel of the mixed-precision solve. */
+ * cycling nbuf sets of its arrays (random contents), HIP events on the
+ * context stream.  The dominant kernel of the mixed-precision solve. */
 int chs_time_cg_stencil(chs_ctx *ctx, int prec, int nbuf, int iters, double hpin, int imass, double *avg_ms);
 
 /* ---- random numbers (rhmc/cheisdwf.f:1678-1801) -------------------------- */
