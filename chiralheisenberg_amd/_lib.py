@@ -88,6 +88,7 @@ def _declare(L):
         "chs_solve_mdagm_dev": ([v, v, v, v, d, i, d, i, d, i, ctypes.POINTER(SolveStats)], i),
         "chs_time_dslash": ([v, i, i, i, i, d, i, ctypes.POINTER(ctypes.c_double)], i),
         "chs_time_cg_stencil": ([v, i, i, i, d, i, ctypes.POINTER(ctypes.c_double)], i),
+        "chs_time_copy": ([v, i, i, i, ctypes.POINTER(ctypes.c_double)], i),
         "chs_nn_init": ([v, i, ctypes.c_ulonglong], i),
         "chs_nn_num_weights": ([v], l),
         "chs_nn_set_weights": ([v, v, l], i),

@@ -1097,6 +1097,46 @@ int chs_time_dslash(chs_ctx *c, int prec, int dagger, int nbuf, int iters, doubl
     return CHS_OK;
 }
 
+// A plain copy of one spinor (2V bytes, non-temporal stores) over nbuf
+// rotating buffer pairs of random contents: the streaming rate the stencil's
+// HBM figure is set beside.
+int chs_time_copy(chs_ctx *c, int prec, int nbuf, int iters, double *avg_ms) {
+    if (prec < 0 || prec > 2 || nbuf < 1 || iters < 1) return fail(c, CHS_EINVAL, "bad arguments");
+    if (c->kferm >= (1L << 32)) return fail(c, CHS_EINVAL, "spinor too large for the copy kernel");
+    const size_t bytes = (size_t)c->kferm * prec_bytes(prec);
+    std::vector<void *> ins(nbuf), outs(nbuf);
+    for (int i = 0; i < nbuf; i++) {
+        HIPCHK(c, hipMalloc(&ins[i], bytes));
+        HIPCHK(c, hipMalloc(&outs[i], bytes));
+        fill_random(c, ins[i], prec, c->kferm, 2 + i);
+    }
+    const long n = c->kferm;
+    const dim3 g(nblocks(n));
+    auto launch = [&](int i) {
+        void *o = outs[i % nbuf], *in = ins[i % nbuf];
+        if (prec == CHS_FP64)
+            hipLaunchKernelGGL(k_copy_nt<F64>, g, dim3(kBlock), 0, c->stream, n, (double2 *)o, (const double2 *)in);
+        else if (prec == CHS_FP32)
+            hipLaunchKernelGGL(k_copy_nt<F32>, g, dim3(kBlock), 0, c->stream, n, (float2 *)o, (const float2 *)in);
+        else
+            hipLaunchKernelGGL(k_copy_nt<B16>, g, dim3(kBlock), 0, c->stream, n, (uint32_t *)o, (const uint32_t *)in);
+    };
+    for (int i = 0; i < nbuf; i++) launch(i);   // warm-up
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < iters; i++) launch(i);
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = ms / iters;
+    for (int i = 0; i < nbuf; i++) {
+        hipFree(ins[i]);
+        hipFree(outs[i]);
+    }
+    return CHS_OK;
+}
+
 // The fused CG stencil of the reliable-update solve (M^dagger applied to M p
 // with r -= alpha M^dagger M p and |r|^2 partials), cycling nbuf sets of its
 // arrays; alpha = 0, so the traffic is that of a solve (r is rewritten with

@@ -54,6 +54,15 @@ __device__ __forceinline__ void store_nt(typename S::st *__restrict__ p, unsigne
     }
 }
 
+// Plain streaming copy of n elements with the stencil's non-temporal stores:
+// the reference rate for the stencil's HBM figures (chs_time_copy).
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_copy_nt(long n, typename S::st *__restrict__ dst,
+                                                    const typename S::st *__restrict__ src) {
+    const long i = logical_block() * kBlock + threadIdx.x;
+    if (i < n) store_nt<S>(dst, (unsigned)i, S::ld(src, i));
+}
+
 // o += coef * (1 - e g_mu) (p0, p1, p2, p3).  MU = 0, 1, 2 (x, y, t) with
 // e = +1; MU = -1, -2, -3 the same with e = -1.
 template <class T, int MU>

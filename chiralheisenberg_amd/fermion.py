@@ -340,6 +340,14 @@ def time_dslash(lat, precision="fp64", dagger=False, nbuf=4, iters=100, hpin=0.0
     return ms.value
 
 
+def time_copy(lat, precision="fp64", nbuf=4, iters=100):
+    """Average ms per plain copy of one spinor (2V bytes, random contents,
+    non-temporal stores), cycling nbuf buffer pairs (HIP events)."""
+    ms = ctypes.c_double()
+    check(lat.h, lib().chs_time_copy(lat.h, PRECISIONS[precision], nbuf, iters, ctypes.byref(ms)))
+    return ms.value
+
+
 def time_cg_stencil(lat, precision="fp32", nbuf=4, iters=100, hpin=0.0, imass=1):
     """Average ms per launch of the solve's fused CG stencil (HIP events)."""
     ms = ctypes.c_double()

@@ -299,6 +299,13 @@ int chs_qmrherm_batch_dev(chs_batch *b, chs_ms_system *sys, int nsys, int prec, 
 int chs_time_dslash(chs_ctx *ctx, int prec, int dagger, int nbuf, int iters,
                     double hpin, int imass, double *avg_ms);
 
+/* Average duration in milliseconds of a plain copy of one spinor of
+ * precision prec (2V bytes; non-temporal stores as in the stencil) cycling
+ * nbuf buffer pairs of random contents, HIP events on the context stream:
+ * the streaming rate the stencil's HBM figure is compared with.  A measuring
+ * helper with no counterpart in the reference. */
+int chs_time_copy(chs_ctx *ctx, int prec, int nbuf, int iters, double *avg_ms);
+
 /* Average duration in milliseconds of the fused CG stencil of
  * chs_solve_mdagm_dev (M^dagger applied to M p, fused with
  * r -= alpha M^dagger M p and the |r|^2 partials; prec CHS_FP64 or CHS_FP32),
