@@ -6,11 +6,13 @@ antiperiodic in t, to FP64 true relative residual <= 1e-10, with phi_a ~
 N(0, 1/beta) and a unit-variance complex Gaussian source drawn by the
 reference's own generator (seeded), inputs resident in HBM.
 
-The timed form is the fastest one the library has (mixed precision with FP64
-reliable updates); the plain FP64 GPU solve of the same system is timed
-beside it, and the dslash kernel's HBM rate is measured with HIP events over
-back-to-back launches that cycle through 4 buffer pairs (> the 256 MB
-Infinity Cache).
+The timed form is the fastest one the library has: CG with the search
+direction and its image stored in binary16 (scaled by the residual's rms),
+the residual and x increment in FP32, and FP64 reliable updates and true
+residual.  The plain FP64 GPU solve and the FP32-storage form of the same
+system are timed beside it, and the dslash kernel's HBM rate is measured
+with HIP events over back-to-back launches that cycle through 4 buffer pairs
+(> the 256 MB Infinity Cache) of random values.
 
 Prints one JSON line (rank 0, N = 1).
 """
@@ -206,7 +208,7 @@ def main():
     ap.add_argument("--Ls", type=int, default=32)
     ap.add_argument("--beta", type=float, default=0.475)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--form", default="fp32", choices=["fp64", "fp32", "bf16"])
+    ap.add_argument("--form", default="fp16", choices=["fp64", "fp32", "bf16", "fp16"])
     ap.add_argument("--delta", type=float, default=0.0, help="reliable-update threshold (0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--counters", action="store_true",
@@ -226,19 +228,20 @@ def main():
     from chiralheisenberg_amd import fermion as fm
     lat = fm.Lattice(args.L, args.Ls)
     if args.counters:
-        # counter runs: plain copies of an FP64 and an FP32 spinor (k_convert:
-        # 16 / 8 B read and written per complex number) calibrate FETCH_SIZE /
+        # counter runs: plain copies of an FP64, an FP32 and a BF16 spinor
+        # (k_convert: 16 / 8 / 4 B read and written per complex number) calibrate FETCH_SIZE /
         # WRITE_SIZE against known bytes; then 20 launches of each roofline
         # kernel over 4 rotating buffer sets
-        for prec in (0, 1):
-            a = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128), ("fp64", "fp32")[prec])
-            b2 = fm.spinor_dev(lat, None, ("fp64", "fp32")[prec])
+        for prec in (0, 1, 2):
+            a = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128), ("fp64", "fp32", "bf16")[prec])
+            b2 = fm.spinor_dev(lat, None, ("fp64", "fp32", "bf16")[prec])
             fm.check(lat.h, fm.lib().chs_convert_dev(lat.h, b2.ptr, prec, a.ptr, prec, lat.kferm))
             lat.synchronize()
         print("bench: dslash fp64 %.2f us" % (fm.time_dslash(lat, "fp64", False, nbuf=4, iters=20) * 1e3),
               flush=True)
-        print("bench: cg stencil fp32 %.2f us" % (fm.time_cg_stencil(lat, "fp32", nbuf=4, iters=20) * 1e3),
-              flush=True)
+        for prec in ("fp32", "fp16"):
+            print("bench: cg stencil %s %.2f us" % (prec, fm.time_cg_stencil(lat, prec, nbuf=4, iters=20) * 1e3),
+                  flush=True)
         return
     chi, b = make_inputs(lat, args.beta)
     chi_d = fm.DeviceArray.from_numpy(lat, chi)
@@ -268,6 +271,15 @@ def main():
     print("bench: %s solve %.1f ms (wall %.1f ms), %d its, %d reliable updates, resid %.2e" % (
         args.form, st["seconds"] * 1e3, wall * 1e3, st["iterations"], st["reliable_updates"], st["true_resid"]),
         file=sys.stderr, flush=True)
+    # the other mixed forms, once each, for comparison
+    others = {}
+    for form in ("fp32", "fp16", "bf16"):
+        if form != args.form:
+            so = solve(form)
+            others[form] = {"ms": so["seconds"] * 1e3, "iterations": so["iterations"],
+                            "reliable_updates": so["reliable_updates"], "true_resid": so["true_resid"]}
+            print("bench: %s solve %.1f ms, %d its" % (form, so["seconds"] * 1e3, so["iterations"]), file=sys.stderr,
+                  flush=True)
 
     # rooflines: the FP64 dslash (2 V64 + C), the kernel the metric names, and
     # the dominant kernel of the timed FP32 solve, its fused CG stencil
@@ -275,20 +287,29 @@ def main():
     kvol = lat.kvol
     V = lat.kferm * 16
     V32 = lat.kferm * 8
+    V16 = lat.kferm * 4
     C = 3 * kvol * 8
-    ms_cg = fm.time_cg_stencil(lat, "fp32", nbuf=4, iters=200)
-    achieved_cg = (3 * V32 + C) / (ms_cg * 1e-3)
+    # the fused CG stencil of the timed form: reads M p (storage precision)
+    # and r, writes r (r in FP64 for the FP64 form, FP32 otherwise)
+    cg_form = args.form
+    cg_bytes = {"fp64": 3 * V + C, "fp32": 3 * V32 + C, "bf16": V16 + 2 * V32 + C, "fp16": V16 + 2 * V32 + C}[cg_form]
+    cg_kernel = {"fp64": "k_dsl<F64,F64,true,EPI_CG>", "fp32": "k_dsl_pair<F32,F32,true,EPI_CG>",
+                 "bf16": "k_dsl<B16,F32,true,EPI_CG>", "fp16": "k_dsl<H16,F32,true,EPI_CG>"}[cg_form]
+    ms_cg = fm.time_cg_stencil(lat, cg_form, nbuf=4, iters=200)
+    achieved_cg = cg_bytes / (ms_cg * 1e-3)
     ms = fm.time_dslash(lat, "fp64", False, nbuf=4, iters=200)
     achieved = (2 * V + C) / (ms * 1e-3)
     ms32 = fm.time_dslash(lat, "fp32", False, nbuf=4, iters=200)
     ms16 = fm.time_dslash(lat, "bf16", False, nbuf=4, iters=200)
+    msh = fm.time_dslash(lat, "fp16", False, nbuf=4, iters=200)
     # the streaming ceiling on the same kind of data: a plain copy of one FP64
     # spinor (2V) with the stencil's non-temporal stores
     ms_copy = fm.time_copy(lat, "fp64", nbuf=4, iters=200)
     copy_rate = 2 * V / (ms_copy * 1e-3)
-    print("bench: cg stencil fp32 %.2f us (%.0f GB/s); dslash fp64 %.2f us (%.0f GB/s), fp32 %.2f us, bf16 %.2f us; "
-          "copy fp64 %.2f us (%.0f GB/s)" % (ms_cg * 1e3, achieved_cg / 1e9, ms * 1e3, achieved / 1e9, ms32 * 1e3,
-                                            ms16 * 1e3, ms_copy * 1e3, copy_rate / 1e9), file=sys.stderr, flush=True)
+    print("bench: cg stencil %s %.2f us (%.0f GB/s); dslash fp64 %.2f us (%.0f GB/s), fp32 %.2f us, bf16 %.2f us, "
+          "fp16 %.2f us; copy fp64 %.2f us (%.0f GB/s)" % (cg_form, ms_cg * 1e3, achieved_cg / 1e9, ms * 1e3,
+                                                          achieved / 1e9, ms32 * 1e3, ms16 * 1e3, msh * 1e3,
+                                                          ms_copy * 1e3, copy_rate / 1e9), file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
@@ -301,7 +322,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32+f64", "bf16": "bf16+f32+f64"}[args.form],
+        "dtype": {"fp64": "f64", "fp32": "f32+f64", "bf16": "bf16+f32+f64", "fp16": "f16+f32+f64"}[args.form],
         "data": "synthetic (seeded phi ~ N(0,1/beta), complex Gaussian source, reference RNG)",
         "config": {"workload": "M^dagger M solve, domain-wall operator, L=Lt=%d Ls=%d beta=%g M5=1 imass=1, "
                                "to FP64 true relative residual %g" % (args.L, args.Ls, args.beta, args.tol),
@@ -312,7 +333,8 @@ def main():
         "fp64_solve": {"ms": st64["seconds"] * 1e3, "iterations": st64["iterations"],
                        "true_resid": st64["true_resid"]},
         "speedup_vs_fp64": st64["seconds"] / max(wall, 1e-12),
-        "dslash_us": {"fp64": ms * 1e3, "fp32": ms32 * 1e3, "bf16": ms16 * 1e3},
+        "other_forms": others,
+        "dslash_us": {"fp64": ms * 1e3, "fp32": ms32 * 1e3, "bf16": ms16 * 1e3, "fp16": msh * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": measured_traffic("dslash", args.L, args.Ls),
                      "kernel": "k_dsl<F64,F64,false,EPI_STORE>", "us": ms * 1e3, "bytes_per_launch": 2 * V + C,
@@ -323,9 +345,9 @@ def main():
                      "frac_of_copy": achieved / copy_rate},
         "roofline_solve_kernel": {"bound": "hbm", "achieved": achieved_cg / 1e9, "peak": HBM_PEAK / 1e9,
                                   "unit": "GB/s", "frac": achieved_cg / HBM_PEAK,
-                                  "traffic": measured_traffic("cg_stencil", args.L, args.Ls),
-                                  "kernel": "k_dsl_pair<F32,F32,true,EPI_CG>", "us": ms_cg * 1e3,
-                                  "bytes_per_launch": 3 * V32 + C,
+                                  "traffic": measured_traffic("cg_stencil_" + cg_form, args.L, args.Ls),
+                                  "kernel": cg_kernel, "us": ms_cg * 1e3,
+                                  "bytes_per_launch": cg_bytes,
                                   "note": "isolated, 4 rotating buffer sets (cache-cold); inside the solve its "
                                           "working set stays in the 256 MB Infinity Cache"},
     }

@@ -8,9 +8,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libchs.so")
 
 CHS_OK, CHS_EINVAL, CHS_EHIP, CHS_ENOCONV, CHS_ENOMEM = 0, 1, 2, 3, 4
-CHS_FP64, CHS_FP32, CHS_BF16 = 0, 1, 2
-PRECISIONS = {"fp64": CHS_FP64, "fp32": CHS_FP32, "bf16": CHS_BF16}
-BYTES_PER_COMPLEX = {CHS_FP64: 16, CHS_FP32: 8, CHS_BF16: 4}
+CHS_FP64, CHS_FP32, CHS_BF16, CHS_FP16 = 0, 1, 2, 3
+PRECISIONS = {"fp64": CHS_FP64, "fp32": CHS_FP32, "bf16": CHS_BF16, "fp16": CHS_FP16}
+BYTES_PER_COMPLEX = {CHS_FP64: 16, CHS_FP32: 8, CHS_BF16: 4, CHS_FP16: 4}
 MAX_SHIFTS = 32
 
 _lib = None

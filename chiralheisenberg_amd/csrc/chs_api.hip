@@ -141,6 +141,7 @@ int d2h_spinor(chs_ctx *c, void *host, const void *src) {
 }
 
 size_t prec_bytes(int prec) { return prec == CHS_FP64 ? 16 : prec == CHS_FP32 ? 8 : 4; }
+bool prec_ok(int prec) { return prec >= CHS_FP64 && prec <= CHS_FP16; }
 
 // n complex numbers of storage precision prec at p <- hashed uniform values in
 // [-0.5, 0.5) (timing inputs with realistic bit patterns; see k_fill_hash)
@@ -148,7 +149,8 @@ void fill_random(chs_ctx *c, void *p, int prec, long n, unsigned long long seed)
     const dim3 g((unsigned)((n + kBlock - 1) / kBlock));
     if (prec == CHS_FP64) hipLaunchKernelGGL(k_fill_hash<F64>, g, dim3(kBlock), 0, c->stream, n, (double2 *)p, seed);
     else if (prec == CHS_FP32) hipLaunchKernelGGL(k_fill_hash<F32>, g, dim3(kBlock), 0, c->stream, n, (float2 *)p, seed);
-    else hipLaunchKernelGGL(k_fill_hash<B16>, g, dim3(kBlock), 0, c->stream, n, (uint32_t *)p, seed);
+    else if (prec == CHS_BF16) hipLaunchKernelGGL(k_fill_hash<B16>, g, dim3(kBlock), 0, c->stream, n, (uint32_t *)p, seed);
+    else hipLaunchKernelGGL(k_fill_hash<H16>, g, dim3(kBlock), 0, c->stream, n, (uint32_t *)p, seed);
 }
 void fill_random_real(chs_ctx *c, double *p, long n, unsigned long long seed) {
     hipLaunchKernelGGL(k_fill_hash_real, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream, n,
@@ -202,7 +204,8 @@ template <bool DAG>
 void dslash_prec(chs_ctx *c, void *out, const void *in, const double *chi, double hpin, int imass, int prec) {
     if (prec == CHS_FP64) dslash_launch<F64, F64, DAG, EPI_STORE>(c, out, in, chi, hpin, imass, epi());
     else if (prec == CHS_FP32) dslash_launch<F32, F32, DAG, EPI_STORE>(c, out, in, chi, hpin, imass, epi());
-    else dslash_launch<B16, F32, DAG, EPI_STORE>(c, out, in, chi, hpin, imass, epi());
+    else if (prec == CHS_BF16) dslash_launch<B16, F32, DAG, EPI_STORE>(c, out, in, chi, hpin, imass, epi());
+    else dslash_launch<H16, F32, DAG, EPI_STORE>(c, out, in, chi, hpin, imass, epi());
 }
 
 // |a|^2 (mode 0) / Re<a,b> (mode 1) / sum a^2 over reals (mode 2) into dst (device)
@@ -356,7 +359,7 @@ int chs_download(chs_ctx *c, void *dst, const void *src, size_t bytes) {
 }
 
 int chs_convert_dev(chs_ctx *c, void *dst, int dp, const void *src, int sp, long n) {
-    if (dp < 0 || dp > 2 || sp < 0 || sp > 2 || n < 0) return fail(c, CHS_EINVAL, "bad precision or size");
+    if (!prec_ok(dp) || !prec_ok(sp) || n < 0) return fail(c, CHS_EINVAL, "bad precision or size");
     dim3 g(nblocks(n)), b(kBlock);
 #define CV(SD, SS)                                                                                           \
     hipLaunchKernelGGL((k_convert<SD, SS>), g, b, 0, c->stream, n, (typename SD::st *)dst,                  \
@@ -369,14 +372,20 @@ int chs_convert_dev(chs_ctx *c, void *dst, int dp, const void *src, int sp, long
     else if (dp == CHS_FP32 && sp == CHS_BF16) CV(F32, B16);
     else if (dp == CHS_BF16 && sp == CHS_FP64) CV(B16, F64);
     else if (dp == CHS_BF16 && sp == CHS_FP32) CV(B16, F32);
-    else CV(B16, B16);
+    else if (dp == CHS_BF16 && sp == CHS_BF16) CV(B16, B16);
+    else if (dp == CHS_FP16 && sp == CHS_FP64) CV(H16, F64);
+    else if (dp == CHS_FP16 && sp == CHS_FP32) CV(H16, F32);
+    else if (dp == CHS_FP64 && sp == CHS_FP16) CV(F64, H16);
+    else if (dp == CHS_FP32 && sp == CHS_FP16) CV(F32, H16);
+    else if (dp == CHS_FP16 && sp == CHS_FP16) CV(H16, H16);
+    else return fail(c, CHS_EINVAL, "no conversion between BF16 and FP16");
 #undef CV
     HIPCHK(c, hipGetLastError());
     return CHS_OK;
 }
 
 int chs_layout_dev(chs_ctx *c, void *dst, const void *src, int prec, int to_native) {
-    if (prec < 0 || prec > 2) return fail(c, CHS_EINVAL, "precision %d not in {0,1,2}", prec);
+    if (!prec_ok(prec)) return fail(c, CHS_EINVAL, "precision %d not in {0,1,2,3}", prec);
     if (dst == src) return fail(c, CHS_EINVAL, "layout conversion cannot work in place");
     const dim3 g((unsigned)((c->kferm + kBlock - 1) / kBlock)), b(kBlock);
     const int dir = to_native ? 0 : 1;
@@ -384,7 +393,7 @@ int chs_layout_dev(chs_ctx *c, void *dst, const void *src, int prec, int to_nati
         hipLaunchKernelGGL((k_relayout<F64>), g, b, 0, c->stream, c->lat, (double2 *)dst, (const double2 *)src, dir);
     else if (prec == CHS_FP32)
         hipLaunchKernelGGL((k_relayout<F32>), g, b, 0, c->stream, c->lat, (float2 *)dst, (const float2 *)src, dir);
-    else
+    else   // BF16 and FP16: both move 32-bit words
         hipLaunchKernelGGL((k_relayout<B16>), g, b, 0, c->stream, c->lat, (uint32_t *)dst, (const uint32_t *)src,
                            dir);
     HIPCHK(c, hipGetLastError());
@@ -394,7 +403,7 @@ int chs_layout_dev(chs_ctx *c, void *dst, const void *src, int prec, int to_nati
 // ---- operators ------------------------------------------------------------------
 int chs_dslash_dev(chs_ctx *c, void *out, const void *in, const double *chi, double hpin, int imass, int dagger,
                    int prec) {
-    if (prec < 0 || prec > 2) return fail(c, CHS_EINVAL, "precision %d not in {0,1,2}", prec);
+    if (!prec_ok(prec)) return fail(c, CHS_EINVAL, "precision %d not in {0,1,2,3}", prec);
     if (out == in) return fail(c, CHS_EINVAL, "dslash output must not alias its input");
     if (dagger) dslash_prec<true>(c, out, in, chi, hpin, imass, prec);
     else dslash_prec<false>(c, out, in, chi, hpin, imass, prec);
@@ -942,10 +951,6 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
     }
     int status = CHS_OK;
     if (rn2 > target2) {
-        // p = rs, xs = 0
-        hipLaunchKernelGGL((k_convert<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, (typename S::st *)ps,
-                           (const typename SA::st *)rs);
-        HIPCHK(c, hipMemsetAsync(xs, 0, n * ba, c->stream));
         CgState h;
         memset(&h, 0, sizeof h);
         h.rnorm2 = rn2;
@@ -954,7 +959,28 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
         double stop_fac = 0.5;
         h.stop2 = stop_fac * target2;
         h.maxiter = maxiter;
-        HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemsetAsync(xs, 0, n * ba, c->stream));
+        if constexpr (S::scaled) {
+            // p = rs / sigma, sigma = rms of the residual (p starts at zero so
+            // that the update's p term adds nothing)
+            h.scaled = 1;
+            h.nelem = (double)n;
+            h.sigma = std::sqrt(rn2 / (double)n);
+            h.xp_r = 1.0 / h.sigma;
+            h.xp_p = 0.0;
+            HIPCHK(c, hipMemsetAsync(ps, 0, n * bs, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+            hipLaunchKernelGGL((k_cg_pupdate<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
+                               (typename S::st *)ps, (const typename SA::st *)rs, (const CgState *)c->cg);
+        } else {
+            // p = rs
+            hipLaunchKernelGGL((k_convert<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, (typename S::st *)ps,
+                               (const typename SA::st *)rs);
+            HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+        }
+        // the fused CG stencil multiplies the stored M^dagger M p by alpha
+        // (alpha sigma for scaled storage)
+        const double *alpha_ptr = S::scaled ? &c->cg->alpha_s : &c->cg->alpha;
         const int *skip = &c->cg->halt;
         const int chunk = chunk_for(c, 11.0 * bs * n);
         // one chunk of iterations: five launches each, replayed from a
@@ -967,7 +993,7 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
                                    (const double *)c->partial, np);
                 // r -= alpha M^dagger M p (the x increment moves to the next kernel)
                 dslash_launch<S, SA, true, EPI_CG>(c, nullptr, mp, chi, hpin, imass,
-                                                  epi(c->partial, &c->cg->alpha, ps, xs, rs, skip, 1));
+                                                  epi(c->partial, alpha_ptr, ps, xs, rs, skip, 1));
                 hipLaunchKernelGGL(k_rcg_beta, dim3(1), dim3(kBlock), 0, c->stream, c->cg,
                                    (const double *)c->partial, np);
                 hipLaunchKernelGGL((k_cg_xpupdate<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
@@ -1024,6 +1050,12 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
             }
             // continue CG from the true residual with the search direction kept
             h.beta = rn2 / h.rnorm2;
+            if constexpr (S::scaled) {
+                const double sig_new = std::sqrt(rn2 / (double)n);
+                h.xp_r = 1.0 / sig_new;
+                h.xp_p = h.beta * h.sigma / sig_new;
+                h.sigma = sig_new;
+            }
             h.rnorm2 = rn2;
             h.maxr2 = rn2;
             h.hold = 0;
@@ -1064,12 +1096,16 @@ int chs_solve_mdagm_dev(chs_ctx *c, const double *b, double *x, const double *ch
         if (delta <= 0) delta = 0.1;
         return rcg_solve<B16, F32>(c, b, x, chi, hpin, imass, tol, delta, maxiter, stats);
     }
-    return fail(c, CHS_EINVAL, "precision %d not in {0,1,2}", prec);
+    if (prec == CHS_FP16) {
+        if (delta <= 0) delta = 0.1;
+        return rcg_solve<H16, F32>(c, b, x, chi, hpin, imass, tol, delta, maxiter, stats);
+    }
+    return fail(c, CHS_EINVAL, "precision %d not in {0,1,2,3}", prec);
 }
 
 // ---- timing helper for the roofline figure ------------------------------------
 int chs_time_dslash(chs_ctx *c, int prec, int dagger, int nbuf, int iters, double hpin, int imass, double *avg_ms) {
-    if (prec < 0 || prec > 2 || nbuf < 1 || iters < 1) return fail(c, CHS_EINVAL, "bad arguments");
+    if (!prec_ok(prec) || nbuf < 1 || iters < 1) return fail(c, CHS_EINVAL, "bad arguments");
     const size_t bytes = (size_t)c->kferm * prec_bytes(prec);
     std::vector<void *> ins(nbuf), outs(nbuf);
     double *chi = nullptr;
@@ -1101,7 +1137,7 @@ int chs_time_dslash(chs_ctx *c, int prec, int dagger, int nbuf, int iters, doubl
 // rotating buffer pairs of random contents: the streaming rate the stencil's
 // HBM figure is set beside.
 int chs_time_copy(chs_ctx *c, int prec, int nbuf, int iters, double *avg_ms) {
-    if (prec < 0 || prec > 2 || nbuf < 1 || iters < 1) return fail(c, CHS_EINVAL, "bad arguments");
+    if (!prec_ok(prec) || nbuf < 1 || iters < 1) return fail(c, CHS_EINVAL, "bad arguments");
     if (c->kferm >= (1L << 32)) return fail(c, CHS_EINVAL, "spinor too large for the copy kernel");
     const size_t bytes = (size_t)c->kferm * prec_bytes(prec);
     std::vector<void *> ins(nbuf), outs(nbuf);
@@ -1118,7 +1154,7 @@ int chs_time_copy(chs_ctx *c, int prec, int nbuf, int iters, double *avg_ms) {
             hipLaunchKernelGGL(k_copy_nt<F64>, g, dim3(kBlock), 0, c->stream, n, (double2 *)o, (const double2 *)in);
         else if (prec == CHS_FP32)
             hipLaunchKernelGGL(k_copy_nt<F32>, g, dim3(kBlock), 0, c->stream, n, (float2 *)o, (const float2 *)in);
-        else
+        else   // BF16 and FP16: 32-bit words
             hipLaunchKernelGGL(k_copy_nt<B16>, g, dim3(kBlock), 0, c->stream, n, (uint32_t *)o, (const uint32_t *)in);
     };
     for (int i = 0; i < nbuf; i++) launch(i);   // warm-up
@@ -1142,24 +1178,29 @@ int chs_time_copy(chs_ctx *c, int prec, int nbuf, int iters, double *avg_ms) {
 // arrays; alpha = 0, so the traffic is that of a solve (r is rewritten with
 // its own random values).
 int chs_time_cg_stencil(chs_ctx *c, int prec, int nbuf, int iters, double hpin, int imass, double *avg_ms) {
-    if ((prec != CHS_FP64 && prec != CHS_FP32) || nbuf < 1 || iters < 1) return fail(c, CHS_EINVAL, "bad arguments");
-    const size_t bytes = (size_t)c->kferm * prec_bytes(prec);
+    if (!prec_ok(prec) || nbuf < 1 || iters < 1) return fail(c, CHS_EINVAL, "bad arguments");
+    // M p and p in the storage precision; x and r in the accumulation one
+    // (FP64 for FP64, FP32 otherwise), as in chs_solve_mdagm_dev
+    const int pacc = prec == CHS_FP64 ? CHS_FP64 : CHS_FP32;
     std::vector<void *> a(4 * nbuf, nullptr);
     double *chi = nullptr;
     HIPCHK(c, hipMalloc(&chi, 24 * c->kvol));
     fill_random_real(c, chi, 3L * c->kvol, 1);
     HIPCHK(c, hipMemsetAsync(c->dscal, 0, sizeof(double), c->stream));   // alpha
     unsigned long long seed = 2;
-    for (auto &p : a) {
-        HIPCHK(c, hipMalloc(&p, bytes));
-        fill_random(c, p, prec, c->kferm, seed++);   // random bits: zeros run ~15% faster
+    for (size_t k = 0; k < a.size(); k++) {
+        const int pk = (k % 4) < 2 ? prec : pacc;
+        HIPCHK(c, hipMalloc(&a[k], (size_t)c->kferm * prec_bytes(pk)));
+        fill_random(c, a[k], pk, c->kferm, seed++);   // random bits: zeros run ~15% faster
     }
     HIPCHK(c, hipGetLastError());
     auto launch = [&](int i) {
         void **s = &a[4 * (i % nbuf)];   // M p, p, x, r
         EpiArgs e = epi(c->partial, c->dscal, s[1], s[2], s[3], nullptr, 1);
         if (prec == CHS_FP64) dslash_launch<F64, F64, true, EPI_CG>(c, nullptr, s[0], chi, hpin, imass, e);
-        else dslash_launch<F32, F32, true, EPI_CG>(c, nullptr, s[0], chi, hpin, imass, e);
+        else if (prec == CHS_FP32) dslash_launch<F32, F32, true, EPI_CG>(c, nullptr, s[0], chi, hpin, imass, e);
+        else if (prec == CHS_BF16) dslash_launch<B16, F32, true, EPI_CG>(c, nullptr, s[0], chi, hpin, imass, e);
+        else dslash_launch<H16, F32, true, EPI_CG>(c, nullptr, s[0], chi, hpin, imass, e);
     };
     for (int i = 0; i < nbuf; i++) launch(i);   // warm-up
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));

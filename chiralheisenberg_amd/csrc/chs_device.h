@@ -9,8 +9,8 @@
 // itself are in chs_stencil.h.  Everything here is elementwise over planes or
 // acts on the scalar state of the solvers.
 //
-// Storage precisions: FP64 (double2), FP32 (float2), BF16 (re, im packed in
-// one 32-bit word).  FP64 storage computes in FP64; FP32 and BF16 storage
+// Storage precisions: FP64 (double2), FP32 (float2), BF16 and FP16 (re, im
+// packed in one 32-bit word).  FP64 storage computes in FP64; the others
 // compute and accumulate in FP32.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -45,6 +45,7 @@ template <typename T> __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
 struct F64 {
     typedef double2 st;
     typedef double rt;
+    static constexpr bool scaled = false;
     static __device__ __forceinline__ cx<double> ld(const st *p, long i) {
         double2 v = p[i];
         return {v.x, v.y};
@@ -54,6 +55,7 @@ struct F64 {
 struct F32 {
     typedef float2 st;
     typedef float rt;
+    static constexpr bool scaled = false;
     static __device__ __forceinline__ cx<float> ld(const st *p, long i) {
         float2 v = p[i];
         return {v.x, v.y};
@@ -77,6 +79,7 @@ __device__ __forceinline__ uint32_t f2bf(float x) {
 struct B16 {
     typedef uint32_t st;
     typedef float rt;
+    static constexpr bool scaled = false;
     static __device__ __forceinline__ cx<float> ld(const st *p, long i) {
         uint32_t v = p[i];
         return {__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u)};
@@ -92,6 +95,27 @@ struct B16 {
     static __device__ __forceinline__ st2 pack2(cx<float> a, cx<float> b) {
         return make_uint2(f2bf(a.r) | (f2bf(a.i) << 16), f2bf(b.r) | (f2bf(b.i) << 16));
     }
+};
+
+// FP16 storage (re, im as IEEE binary16 in one 32-bit word, round to nearest
+// even), FP32 compute.  FP16 keeps 11 significant bits (BF16: 8) but only 5
+// exponent bits, so the CG that stores p and M p this way keeps them scaled
+// by the current residual's rms (scaled = true: CgState.sigma).
+struct H16 {
+    typedef uint32_t st;
+    typedef float rt;
+    static constexpr bool scaled = true;
+    static __device__ __forceinline__ float h2f(uint32_t b) {
+        return (float)__builtin_bit_cast(_Float16, (unsigned short)b);
+    }
+    static __device__ __forceinline__ uint32_t f2h(float x) {
+        return (uint32_t)__builtin_bit_cast(unsigned short, (_Float16)x);
+    }
+    static __device__ __forceinline__ cx<float> ld(const st *p, long i) {
+        const uint32_t v = p[i];
+        return {h2f(v & 0xffffu), h2f(v >> 16)};
+    }
+    static __device__ __forceinline__ void sto(st *p, long i, cx<float> v) { p[i] = f2h(v.r) | (f2h(v.i) << 16); }
 };
 
 // Blocks are handed to the 8 XCDs round-robin; give each XCD a contiguous
@@ -565,6 +589,11 @@ __global__ __launch_bounds__(kBlock) void k_force_init(long n, double beta, cons
 // iterations ahead and poll.
 struct CgState {
     double alpha, alphan, betad, beta;
+    // scaled storage (H16): p and M p are held divided by sigma, the rms of
+    // the residual when p was formed; alpha_s = alpha * sigma multiplies the
+    // stored M^dagger M p, and the x/p update is x += xp_a p, p = xp_r r +
+    // xp_p p (xp_a = alpha sigma, xp_r = 1 / sigma', xp_p = beta sigma / sigma')
+    double sigma, alpha_s, xp_a, xp_r, xp_p, nelem;
     double resid;        // reference rule: stop when |r|^2 < resid
     double rnorm2;       // |r|^2 of the current iterated residual
     double maxr2;        // |r|^2 at the last reliable update
@@ -576,7 +605,7 @@ struct CgState {
     int halt;            // done | hold
     int iter;            // iterations taken
     int maxiter;
-    int pad;
+    int scaled;          // p and M p are stored scaled (H16)
 };
 
 // reference CGNR: alpha = alphan / |M p|^2
@@ -616,10 +645,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_pupdate(long n, typename S::st *_
     const long i = logical_block() * kBlock + threadIdx.x;
     typedef typename S::rt T;
     if (i < n) {
-        const T b = (T)st->beta;
         const cx<typename SA::rt> rv = SA::ld(r, i);
         const cx<T> pv = S::ld(p, i);
-        S::sto(p, i, cx<T>{(T)rv.r + b * pv.r, (T)rv.i + b * pv.i});
+        if constexpr (S::scaled) {
+            const T cr = (T)st->xp_r, cp = (T)st->xp_p;
+            S::sto(p, i, cx<T>{cr * (T)rv.r + cp * pv.r, cr * (T)rv.i + cp * pv.i});
+        } else {
+            const T b = (T)st->beta;
+            S::sto(p, i, cx<T>{(T)rv.r + b * pv.r, (T)rv.i + b * pv.i});
+        }
     }
 }
 
@@ -636,14 +670,22 @@ __global__ __launch_bounds__(kBlock) void k_cg_xpupdate(long n, typename S::st *
     typedef typename S::rt T;
     typedef typename SA::rt TA;
     if (i < n) {
-        const T b = (T)st->beta;
-        const TA a = (TA)st->alpha;
         const cx<TA> rv = SA::ld(r, i);
         const cx<T> pv = S::ld(p, i);
         cx<TA> xv = SA::ld(x, i);
-        xv = {xv.r + a * (TA)pv.r, xv.i + a * (TA)pv.i};
-        SA::sto(x, i, xv);
-        S::sto(p, i, cx<T>{(T)rv.r + b * pv.r, (T)rv.i + b * pv.i});
+        if constexpr (S::scaled) {
+            const TA a = (TA)st->xp_a;
+            const T cr = (T)st->xp_r, cp = (T)st->xp_p;
+            xv = {xv.r + a * (TA)pv.r, xv.i + a * (TA)pv.i};
+            SA::sto(x, i, xv);
+            S::sto(p, i, cx<T>{cr * (T)rv.r + cp * pv.r, cr * (T)rv.i + cp * pv.i});
+        } else {
+            const T b = (T)st->beta;
+            const TA a = (TA)st->alpha;
+            xv = {xv.r + a * (TA)pv.r, xv.i + a * (TA)pv.i};
+            SA::sto(x, i, xv);
+            S::sto(p, i, cx<T>{(T)rv.r + b * pv.r, (T)rv.i + b * pv.i});
+        }
     }
 }
 
@@ -656,7 +698,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_xupdate(long n, const typename S:
     const long i = logical_block() * kBlock + threadIdx.x;
     typedef typename SA::rt TA;
     if (i < n) {
-        const TA a = (TA)st->alpha;
+        const TA a = (TA)(S::scaled ? st->alpha_s : st->alpha);
         const cx<typename S::rt> pv = S::ld(p, i);
         cx<TA> xv = SA::ld(x, i);
         SA::sto(x, i, cx<TA>{xv.r + a * (TA)pv.r, xv.i + a * (TA)pv.i});
@@ -670,15 +712,24 @@ __global__ __launch_bounds__(kBlock) void k_cg_xupdate(long n, const typename S:
 // memory round trip instead of three: halt flag, partials, then the fields)
 // and test halt only before storing.
 __global__ void k_rcg_alpha(CgState *st, const double *__restrict__ partial, int np) {
-    const int halt = st->halt;
-    const double rn = st->rnorm2;
+    const int halt = st->halt, scaled = st->scaled;
+    const double rn = st->rnorm2, sig = st->sigma;
     const double t = sum_partials(partial, np);
-    if (threadIdx.x == 0 && !halt) st->alpha = rn / t;
+    if (threadIdx.x != 0 || halt) return;
+    if (scaled) {
+        // t = |M p|^2 / sigma^2
+        const double a = rn / (sig * sig * t);
+        st->alpha = a;
+        st->alpha_s = a * sig;
+    } else {
+        st->alpha = rn / t;
+    }
 }
 
 __global__ void k_rcg_beta(CgState *st, const double *__restrict__ partial, int np) {
-    const int halt = st->halt, iter = st->iter + 1, maxiter = st->maxiter;
+    const int halt = st->halt, iter = st->iter + 1, maxiter = st->maxiter, scaled = st->scaled;
     const double rn_old = st->rnorm2, d2 = st->delta2, maxr2 = st->maxr2, stop2 = st->stop2;
+    const double sig = st->sigma, alpha_s = st->alpha_s, nelem = st->nelem;
     const double rn = sum_partials(partial, np);
     if (threadIdx.x != 0 || halt) return;
     st->iter = iter;
@@ -687,8 +738,16 @@ __global__ void k_rcg_beta(CgState *st, const double *__restrict__ partial, int 
         st->halt = 1;
         st->rn_hold = rn;
     } else {
-        st->beta = rn / rn_old;
+        const double beta = rn / rn_old;
+        st->beta = beta;
         st->rnorm2 = rn;
+        if (scaled) {
+            const double sig_new = sqrt(rn / nelem);
+            st->xp_a = alpha_s;
+            st->xp_r = 1.0 / sig_new;
+            st->xp_p = beta * sig / sig_new;
+            st->sigma = sig_new;
+        }
     }
 }
 

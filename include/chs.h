@@ -46,7 +46,7 @@ enum {
 };
 
 /* storage precision of a device spinor */
-enum { CHS_FP64 = 0, CHS_FP32 = 1, CHS_BF16 = 2 };
+enum { CHS_FP64 = 0, CHS_FP32 = 1, CHS_BF16 = 2, CHS_FP16 = 3 };
 
 /* Upper bound on the number of rational shifts in one solve. */
 #define CHS_MAX_SHIFTS 32
@@ -179,8 +179,8 @@ int chs_convert_dev(chs_ctx *ctx, void *dst, int dst_prec, const void *src,
                     int src_prec, long ncomplex);
 
 /* out = M in (dagger=0) or M^dagger in (dagger=1) on device spinors of
- * storage precision prec (FP64: double2, FP32: float2, BF16: bf16 re,im
- * pair).  chi is a device real*8 (kvol,3) array. */
+ * storage precision prec (FP64: double2, FP32: float2, BF16 / FP16: bf16 or
+ * IEEE binary16 re,im pair).  chi is a device real*8 (kvol,3) array. */
 int chs_dslash_dev(chs_ctx *ctx, void *out, const void *in, const double *chi,
                    double hpin, int imass, int dagger, int prec);
 
@@ -207,9 +207,11 @@ typedef struct {
 
 /* Solve M^dagger M x = b on device arrays to the FP64 true relative
  * residual tol.  prec selects the storage precision of the inner CG
- * (CHS_FP64: plain FP64 CG; CHS_FP32 / CHS_BF16: reliable-update CG with
- * FP64 true residuals every time the inner residual has dropped by delta,
- * delta <= 0 picks a default).  x holds the initial guess on entry. */
+ * (CHS_FP64: plain FP64 CG; CHS_FP32 / CHS_BF16 / CHS_FP16: reliable-update
+ * CG with FP64 true residuals every time the inner residual has dropped by
+ * delta, delta <= 0 picks a default; in CHS_FP16 the search direction and its
+ * image are stored in binary16 divided by the residual's rms, the residual
+ * and the x increment in FP32).  x holds the initial guess on entry. */
 int chs_solve_mdagm_dev(chs_ctx *ctx, const double *b, double *x,
                         const double *chi, double hpin, int imass, double tol,
                         int prec, double delta, int maxiter,
@@ -308,7 +310,8 @@ int chs_time_copy(chs_ctx *ctx, int prec, int nbuf, int iters, double *avg_ms);
 
 /* Average duration in milliseconds of the fused CG stencil of
  * chs_solve_mdagm_dev (M^dagger applied to M p, fused with
- * r -= alpha M^dagger M p and the |r|^2 partials; prec CHS_FP64 or CHS_FP32),
+ * r -= alpha M^dagger M p and the |r|^2 partials; any storage precision, with
+ * x and r in FP64 for CHS_FP64 and in FP32 otherwise),
  * cycling nbuf sets of its arrays (random contents), HIP events on the
  * context stream.  The dominant kernel of the mixed-precision solve. */
 int chs_time_cg_stencil(chs_ctx *ctx, int prec, int nbuf, int iters, double hpin, int imass, double *avg_ms);

@@ -59,12 +59,13 @@ def test_am_has_no_effect():
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("prec,tol", [("fp32", 2e-6), ("bf16", 2e-2)])
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-6), ("bf16", 2e-2), ("fp16", 2e-3)])
 @pytest.mark.parametrize("dagger", [False, True])
 def test_dslash_low_precision(prec, tol, dagger):
     """FP32 storage/compute: within a few FP32 ulps of the operator norm;
     BF16 storage (8-bit mantissa) with FP32 accumulation: within ~2^-7
-    relative in the 2-norm."""
+    relative in the 2-norm; FP16 storage (11-bit significand): within
+    ~2^-10."""
     lat, olat = lattices(12, 24, 12)
     rs = np.random.RandomState(7)
     R, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
@@ -102,7 +103,7 @@ def test_dslash_vs_reference_golden_gpu():
             assert abs(np.linalg.norm(out) - n) <= 1e-14 * n
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16", "fp16"])
 def test_native_layout_round_trip(prec):
     """reference -> native -> reference reorders exactly (bit-exact in FP64;
     in FP32/BF16 equal to the rounded input)."""
@@ -114,6 +115,14 @@ def test_native_layout_round_trip(prec):
         assert np.array_equal(back, R)
     elif prec == "fp32":
         assert np.array_equal(back, R.astype(np.complex64).astype(np.complex128))
+    elif prec == "fp16":
+        # round to nearest even, either directly from FP64 (what the compiler
+        # makes of the FP64 -> FP32 -> binary16 conversion) or via FP32
+        for part in (np.real, np.imag):
+            v, got = part(R), part(back)
+            one = v.astype(np.float16).astype(np.float64)
+            two = v.astype(np.float32).astype(np.float16).astype(np.float64)
+            assert np.all((got == one) | (got == two))
     else:
         assert np.abs(back - R).max() <= 2 ** -8 * np.abs(R).max()
 

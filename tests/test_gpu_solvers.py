@@ -114,7 +114,7 @@ def test_hamilton_parity():
         assert abs(a - b) <= 1e-10 * abs(b), (hg, hc)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16", "fp16"])
 def test_solve_mdagm_true_residual(prec):
     """Every form reaches an FP64 true relative residual <= 1e-10, checked
     here independently with the oracle's operator."""
