@@ -184,19 +184,24 @@ struct EpiArgs {
 
 // ---- deterministic FP64 reductions -----------------------------------------
 // One workgroup sums n partials in a fixed order: thread t adds part[t],
-// part[t + kBlock], ... in that order, then block_sum.  The loads are issued
-// eight at a time ahead of the additions (the sum is latency-bound).
+// part[t + kBlock], ... in that order, then block_sum.  The sum is
+// latency-bound: up to 16 loads per thread are issued (predicated on the
+// bound) before the first addition, so n <= 16 * kBlock partials cost one
+// memory round trip.
 __device__ __forceinline__ double sum_partials(const double *__restrict__ part, int n) {
+    constexpr int kU = 16;
     double v = 0.0;
-    int i = threadIdx.x;
-    for (; i + 7 * kBlock < n; i += 8 * kBlock) {
-        double t[8];
+    for (int i = threadIdx.x; i < n; i += kU * kBlock) {
+        double t[kU];
 #pragma unroll
-        for (int k = 0; k < 8; k++) t[k] = part[i + k * kBlock];
+        for (int k = 0; k < kU; k++) {
+            const int j = i + k * kBlock;
+            t[k] = j < n ? part[j] : 0.0;
+        }
 #pragma unroll
-        for (int k = 0; k < 8; k++) v += t[k];
+        for (int k = 0; k < kU; k++)
+            if (i + k * kBlock < n) v += t[k];
     }
-    for (; i < n; i += kBlock) v += part[i];
     return block_sum(v);
 }
 
