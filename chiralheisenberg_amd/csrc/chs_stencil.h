@@ -92,6 +92,18 @@ __device__ __forceinline__ void hop_vals(cx<T> (&o)[4], cx<T> p0, cx<T> p1, cx<T
     o[3] = {o[3].r + coef * h3.r, o[3].i + coef * h3.i};
 }
 
+// |a|^2 and Re <a, b> formed in FP64 whatever the compute type: the epilogue
+// partials of FP32 / 16-bit storage would otherwise under- or overflow for
+// components beyond ~1e-19 .. 1e19 (squares outside FP32's range).
+template <class U>
+__device__ __forceinline__ double norm2_d(cx<U> a) {
+    return (double)a.r * (double)a.r + (double)a.i * (double)a.i;
+}
+template <class U>
+__device__ __forceinline__ double dotre_d(cx<U> a, cx<U> b) {
+    return (double)a.r * (double)b.r + (double)a.i * (double)b.i;
+}
+
 // o[ff] += coef * (1 - e g_mu) psi_ff(gn); pl[ff][d] are uniform plane bases.
 template <class S, int MU, int NF>
 __device__ __forceinline__ void hop2(cx<typename S::rt> (&o)[NF][4], const typename S::st *const (&pl)[NF][4],
@@ -263,7 +275,7 @@ __device__ __forceinline__ void dsl_block(const Lat &lat, typename S::st *__rest
         }
         cx<T> o[1][4], c[1][4];
         stencil<S, DAG, 1, LAYOUT>(lat, in, chi, hpin, imass, s, site, fb, o, c);
-        T acc = 0;
+        double acc = 0.0;
 #pragma unroll
         for (int d = 0; d < 4; d++) {
             const size_t po = P * (size_t)(d + 4 * fb);
@@ -285,14 +297,14 @@ __device__ __forceinline__ void dsl_block(const Lat &lat, typename S::st *__rest
                 cx<TA> rv = SA::ld(rr, g);
                 rv = {rv.r - alpha * (TA)o[0][d].r, rv.i - alpha * (TA)o[0][d].i};
                 store_nt<SA>(rr, g, rv);
-                acc += rv.r * rv.r + rv.i * rv.i;
+                acc += norm2_d(rv);
             } else {
                 store_nt<S>(out + po, g, o[0][d]);
-                if (EPI == EPI_NORM) acc += o[0][d].r * o[0][d].r + o[0][d].i * o[0][d].i;
-                if (EPI == EPI_DOT) acc += apre[d].r * o[0][d].r + apre[d].i * o[0][d].i;
+                if (EPI == EPI_NORM) acc += norm2_d(o[0][d]);
+                if (EPI == EPI_DOT) acc += dotre_d(apre[d], o[0][d]);
             }
         }
-        part = (double)acc;
+        part = acc;
     }
     if (EPI == EPI_NORM || EPI == EPI_DOT || EPI == EPI_CG || EPI == EPI_CGQ) {
         const double t = block_sum(part);
@@ -455,7 +467,7 @@ __device__ __forceinline__ void dsl_block_pair(const Lat &lat, typename S::st *_
         }
         cx<T> o0[4], o1[4], c0[4], c1[4];
         stencil_pair<S, DAG>(lat, in, chi, hpin, imass, s, site, fb, o0, o1, c0, c1);
-        T acc = 0;
+        double acc = 0.0;
 #pragma unroll
         for (int d = 0; d < 4; d++) {
             const size_t po = P * (size_t)(d + 4 * fb);
@@ -482,16 +494,16 @@ __device__ __forceinline__ void dsl_block_pair(const Lat &lat, typename S::st *_
                 ra = {ra.r - alpha * (TA)o0[d].r, ra.i - alpha * (TA)o0[d].i};
                 rb = {rb.r - alpha * (TA)o1[d].r, rb.i - alpha * (TA)o1[d].i};
                 store_nt2<SA>(rr, g, SA::pack2(ra, rb));
-                acc += ra.r * ra.r + ra.i * ra.i + rb.r * rb.r + rb.i * rb.i;
+                acc += norm2_d(ra) + norm2_d(rb);
             } else {
                 store_nt2<S>(out + po, g, S::pack2(o0[d], o1[d]));
                 if (EPI == EPI_NORM)
-                    acc += o0[d].r * o0[d].r + o0[d].i * o0[d].i + o1[d].r * o1[d].r + o1[d].i * o1[d].i;
+                    acc += norm2_d(o0[d]) + norm2_d(o1[d]);
                 if (EPI == EPI_DOT)
-                    acc += apa[d].r * o0[d].r + apa[d].i * o0[d].i + apb[d].r * o1[d].r + apb[d].i * o1[d].i;
+                    acc += dotre_d(apa[d], o0[d]) + dotre_d(apb[d], o1[d]);
             }
         }
-        part = (double)acc;
+        part = acc;
     }
     if (EPI == EPI_NORM || EPI == EPI_DOT || EPI == EPI_CG || EPI == EPI_CGQ) {
         const double t = block_sum(part);

@@ -114,6 +114,30 @@ def test_hamilton_parity():
         assert abs(a - b) <= 1e-10 * abs(b), (hg, hc)
 
 
+@pytest.mark.parametrize("scale", [1e-20, 1e20])
+def test_solve_mdagm_fp16_scaled_source(scale):
+    """The binary16 form stores p and M p divided by the residual's rms: a
+    source 1e-20 or 1e+20 times a unit one (far outside binary16's 6e-8 ..
+    65504) reaches the same FP64 true relative residual, and the solution is
+    the scaled solution of the unit source (to what two 1e-10 solves
+    allow)."""
+    lat, olat = lattices(8, 8)
+    rs = np.random.RandomState(72)
+    b, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    chid = fm.DeviceArray.from_numpy(lat, chi)
+    xs = []
+    for sc in (1.0, scale):
+        bd = fm.spinor_dev(lat, sc * b)
+        xd = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+        st = fm.solve_mdagm(lat, bd, xd, chid, 1e-10, "fp16")
+        x = fm.to_host(lat, xd)
+        r = sc * b - orc.dslash(olat, orc.dslash(olat, x, chi, 0.0, 1, False), chi, 0.0, 1, True)
+        rel = np.linalg.norm(r) / np.linalg.norm(sc * b)
+        assert rel <= 1.05e-10, (sc, rel, st)
+        xs.append(x / sc)
+    assert np.linalg.norm(xs[1] - xs[0]) <= 5e-6 * np.linalg.norm(xs[0])
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16", "fp16"])
 def test_solve_mdagm_true_residual(prec):
     """Every form reaches an FP64 true relative residual <= 1e-10, checked
