@@ -488,6 +488,13 @@ int qmrherm_impl(chs_ctx *c, const double *phi, double res, int *itercg, double 
     const int np1 = (int)dsl_grid<S>(c), np2 = (int)nblocks(n);
     const int chunk = chunk_for(c, (12.0 + 2.0 * ndiag) * (double)sizeof(st) * n);
     const int *skip = &c->lz->done;
+    // the force keeps its x_k only on the slices k_force_shift reads
+    XKeep xkeep;
+    if (iflag == 1 && c->Ls > 4) {
+        xkeep.n5 = (unsigned)c->n5;
+        xkeep.lo = 2u * (unsigned)c->kvol;
+        xkeep.hi = (unsigned)(c->Ls - 2) * (unsigned)c->kvol;
+    }
     int done = 0, iter = 1;
     while (iter <= niterc && !done) {
         const int last = std::min(niterc, iter + chunk - 1);
@@ -503,7 +510,7 @@ int qmrherm_impl(chs_ctx *c, const double *phi, double res, int *itercg, double 
             const LzState *lzc = c->lz;
 #define LZ_UPD(NS, CMB)                                                                                         \
     hipLaunchKernelGGL((k_lz_update<S, NS, CMB>), dim3(np2), dim3(kBlock), 0, c->stream, n, (const st *)w,      \
-                       (const st *)qc, (const st *)qp, r, pk, xk, an, lzc, ndiag, iter, c->partial2)
+                       (const st *)qc, (const st *)qp, r, pk, xk, an, lzc, ndiag, iter, c->partial2, xkeep)
             if (iflag == 0) {
                 if (ndiag == 25) LZ_UPD(25, true);
                 else if (ndiag == 12) LZ_UPD(12, true);

@@ -389,6 +389,21 @@ __device__ __forceinline__ void lz_coefs(LzCoef &sc, const LzState *__restrict__
     __syncthreads();
 }
 
+// Which elements of the x_k the shift update keeps (CMB = false): those whose
+// offset inside their plane, i mod n5, lies outside [lo, hi).  The force
+// (rhmc/cheisdwf.f:733-849) reads x_k only on the slices s = 0, 1, Ls-2,
+// Ls-1 (the wall slices and their s-neighbours: derivs uses y = x_k and
+// z = M x_k at s in {1, Ls} only), so it passes lo = 2 kvol, hi = (Ls-2) kvol
+// and the other slices of every x_k are neither read nor written.  The
+// default keeps everything.
+struct XKeep {
+    unsigned n5 = 1, lo = 0, hi = 0;
+    __device__ __forceinline__ bool keep(long i) const {
+        const unsigned e = (unsigned)i % n5;
+        return e < lo || e >= hi;
+    }
+};
+
 // Storage S of r, q, w, p_k (and x_k); arithmetic in FP64 on loaded values,
 // rounded on store.  With CMB the running combination x is FP64 (double2).
 template <class S, int NS, bool CMB>
@@ -397,7 +412,7 @@ __device__ __forceinline__ double lz_update_elem(long i, long n, const typename 
                                                  const typename S::st *__restrict__ qm1,
                                                  typename S::st *__restrict__ r, typename S::st *__restrict__ p,
                                                  void *__restrict__ xp, const LzState *__restrict__ st,
-                                                 const LzCoef &sc, int ndiag, int iter) {
+                                                 const LzCoef &sc, int ndiag, int iter, XKeep xkeep = XKeep()) {
     typedef cx<typename S::rt> C;
     const double at = st->alphatild, b0 = st->beta;
     const C qs = S::ld(q, i), ws = S::ld(w, i), qms = S::ld(qm1, i);
@@ -432,14 +447,16 @@ __device__ __forceinline__ double lz_update_elem(long i, long n, const typename 
         x[i] = xv;
     } else {
         typename S::st *x = (typename S::st *)xp;
+        const bool kx = xkeep.keep(i);
         if (iter == 1) {
 #pragma unroll 4
             for (int k = 0; k < ns; k++) {
                 S::sto(p, (long)k * n + i, qs);
-                S::sto(x, (long)k * n + i,
-                       C{(typename S::rt)(sc.rho[k] * qv.x), (typename S::rt)(sc.rho[k] * qv.y)});
+                if (kx)
+                    S::sto(x, (long)k * n + i,
+                           C{(typename S::rt)(sc.rho[k] * qv.x), (typename S::rt)(sc.rho[k] * qv.y)});
             }
-        } else {
+        } else if (kx) {
 #pragma unroll 4
             for (int k = 0; k < ns; k++) {
                 const C ps = S::ld(p, (long)k * n + i), xs = S::ld(x, (long)k * n + i);
@@ -450,6 +467,15 @@ __device__ __forceinline__ double lz_update_elem(long i, long n, const typename 
                 xv.y = (double)xs.i + sc.rho[k] * pv.y;
                 S::sto(p, (long)k * n + i, C{(typename S::rt)pv.x, (typename S::rt)pv.y});
                 S::sto(x, (long)k * n + i, C{(typename S::rt)xv.x, (typename S::rt)xv.y});
+            }
+        } else {
+#pragma unroll 4
+            for (int k = 0; k < ns; k++) {
+                const C ps = S::ld(p, (long)k * n + i);
+                double2 pv;
+                pv.x = qv.x - sc.amu[k] * (double)ps.r;
+                pv.y = qv.y - sc.amu[k] * (double)ps.i;
+                S::sto(p, (long)k * n + i, C{(typename S::rt)pv.x, (typename S::rt)pv.y});
             }
         }
     }
@@ -465,7 +491,7 @@ __device__ __forceinline__ void lz_update(long n, const typename S::st *__restri
                                           const typename S::st *__restrict__ qm1, typename S::st *__restrict__ r,
                                           typename S::st *__restrict__ p, void *__restrict__ x,
                                           const double *__restrict__ anum, const LzState *__restrict__ st, int ndiag,
-                                          int iter, double *__restrict__ partial, long lb) {
+                                          int iter, double *__restrict__ partial, long lb, XKeep xkeep = XKeep()) {
     __shared__ LzCoef sc;
     lz_coefs<CMB>(sc, st, ndiag, anum);
 #pragma unroll
@@ -473,7 +499,7 @@ __device__ __forceinline__ void lz_update(long n, const typename S::st *__restri
         const long eb = lb * EPT + e;
         const long i = eb * kBlock + threadIdx.x;
         double part = 0.0;
-        if (i < n) part = lz_update_elem<S, NS, CMB>(i, n, w, q, qm1, r, p, x, st, sc, ndiag, iter);
+        if (i < n) part = lz_update_elem<S, NS, CMB>(i, n, w, q, qm1, r, p, x, st, sc, ndiag, iter, xkeep);
         const double t = block_sum(part);
         if (threadIdx.x == 0) partial[eb] = t;
     }
@@ -486,9 +512,9 @@ __global__ __launch_bounds__(kBlock) void k_lz_update(long n, const typename S::
                                                       typename S::st *__restrict__ r, typename S::st *__restrict__ p,
                                                       void *__restrict__ x, const double *__restrict__ anum,
                                                       const LzState *__restrict__ st, int ndiag, int iter,
-                                                      double *__restrict__ partial) {
+                                                      double *__restrict__ partial, XKeep xkeep) {
     if (st->done) return;
-    lz_update<S, NS, CMB, 1>(n, w, q, qm1, r, p, x, anum, st, ndiag, iter, partial, logical_block());
+    lz_update<S, NS, CMB, 1>(n, w, q, qm1, r, p, x, anum, st, ndiag, iter, partial, logical_block(), xkeep);
 }
 
 // ---- propagator measurement (rhmc/cheisdwf.f:897-1058) -------------------
