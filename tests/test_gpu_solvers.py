@@ -114,6 +114,36 @@ def test_hamilton_parity():
         assert abs(a - b) <= 1e-10 * abs(b), (hg, hc)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_solve_mdagm_deterministic(prec, monkeypatch):
+    """The reductions sum in a fixed order, so a solve is reproducible bit for
+    bit: twice in one context (iteration chunks replayed from a hipGraph) and
+    once in a context that launches them one by one (CHS_GRAPHS=0)."""
+    rs = np.random.RandomState(73)
+    b, chi = spinor(rs, (2, 4, 8, 8, 8, 8)), phi_field(rs, 512)
+    xs, its = [], []
+
+    def solve(lat):
+        bd = fm.spinor_dev(lat, b)
+        xd = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+        st = fm.solve_mdagm(lat, bd, xd, fm.DeviceArray.from_numpy(lat, chi), 1e-10, prec)
+        xs.append(fm.to_host(lat, xd))
+        its.append((st["iterations"], st["reliable_updates"]))
+
+    monkeypatch.setenv("CHS_GRAPHS", "1")
+    lat = fm.Lattice(8, 8)
+    assert lat.shape == b.shape
+    solve(lat)
+    solve(lat)
+    lat.close()
+    monkeypatch.setenv("CHS_GRAPHS", "0")
+    lat = fm.Lattice(8, 8)
+    solve(lat)
+    lat.close()
+    assert its[0] == its[1] == its[2], its
+    assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
+
+
 @pytest.mark.parametrize("scale", [1e-20, 1e20])
 def test_solve_mdagm_fp16_scaled_source(scale):
     """The binary16 form stores p and M p divided by the residual's rms: a
