@@ -650,14 +650,16 @@ int chs_congrad_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
 // fermion (rhmc/cheisdwf.f:897-1058): five wall sources at t-slices
 // mod(ittt0 + k (Lt/5), Lt) + 1, Dirac 3 and 4, flavour 1, 1/L^2 at s = 1;
 // x = (M^dagger M)^{-1} M^dagger source from the initial guess M^dagger
-// source (congrad, or the FP32 reliable-update solve to the same residual);
+// source (congrad, or the FP32 / scaled-FP16 reliable-update solve to the same
+// residual);
 // correlators from the t-slice sums of x at s = Ls (cferm1, Dirac = source)
 // and s = 1 (cferm2, gamma_3 partner), with the antiperiodic sign for slices
 // behind the source.
 int chs_fermion(chs_ctx *c, const double *chi, double res, double hpin, int imass, int ittt0, int prec,
                 double *cferm1, double *cferm2, int *iterations) {
     if (ittt0 < 1 || ittt0 > c->Lt) return fail(c, CHS_EINVAL, "ittt0=%d outside 1..%d", ittt0, c->Lt);
-    if (prec != CHS_FP64 && prec != CHS_FP32) return fail(c, CHS_EINVAL, "fermion precision must be FP64 or FP32");
+    if (prec != CHS_FP64 && prec != CHS_FP32 && prec != CHS_FP16)
+        return fail(c, CHS_EINVAL, "fermion precision must be FP64, FP32 or FP16");
     const int nsource = 5, Lt = c->Lt, Ls = c->Ls;
     const long n = c->kferm, L2 = (long)c->L * c->L;
     int rc = ensure_spinors(c, {B_PHI, B_X, B_Q0});
@@ -690,7 +692,7 @@ int chs_fermion(chs_ctx *c, const double *chi, double res, double hpin, int imas
                 HIPCHK(c, hipStreamSynchronize(c->stream));
                 chs_solve_stats st;
                 rc = chs_solve_mdagm_dev(c, (const double *)phi, (double *)x, dchi, hpin, imass,
-                                         std::sqrt((double)n * res * res / b2), CHS_FP32, 0.0, 0, &st);
+                                         std::sqrt((double)n * res * res / b2), prec, 0.0, 0, &st);
                 it = st.iterations;
             }
             if (rc == CHS_ENOCONV) status = rc;   // the reference carries on with the last iterate

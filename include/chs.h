@@ -135,7 +135,8 @@ int chs_hamilton(chs_ctx *ctx, const double *phi, double *h, double *hg,
  * chi explicit and ittt0 = int(Lt*rano(yran,idum)) + 1 (the one random draw
  * of the routine, 1..Lt) passed in.  Five wall sources, Dirac 3 and 4,
  * congrad each (prec = CHS_FP64: the reference CGNR and stopping rule;
- * CHS_FP32: the FP32 reliable-update solve to the same residual).
+ * CHS_FP32 / CHS_FP16: the FP32 or scaled-binary16 reliable-update solve to
+ * the same residual).
  * cferm1 = fort.5001/5002, cferm2 = fort.5011/5012: complex (re, im)
  * [flavour][t], t fastest, 2*Lt complex each.  iterations = total solver
  * iterations (the reference's aviter times 10). */

@@ -50,11 +50,12 @@ def test_fermion_vs_oracle(L, Ls, Lt, imass, ittt0):
     assert np.abs(c1 - o1).max() <= 1e-8 * np.abs(o1).max()
     assert np.abs(c2 - o2).max() <= 1e-8 * np.abs(o2).max()
     assert abs(it - oit) <= 20
-    # FP32 reliable-update solves to the same stopping residual
-    f1, f2, fit = fm.fermion(lat, chi, 1e-11, ittt0, 0.3, imass, precision="fp32")
-    e = max(np.abs(f1 - o1).max() / np.abs(o1).max(), np.abs(f2 - o2).max() / np.abs(o2).max())
-    print("L=%d Ls=%d: fp32 correlator error %.2e, iterations %d vs %d" % (L, Ls, e, fit, it))
-    assert e <= 1e-6
+    # FP32 and scaled-FP16 reliable-update solves to the same stopping residual
+    for prec in ("fp32", "fp16"):
+        f1, f2, fit = fm.fermion(lat, chi, 1e-11, ittt0, 0.3, imass, precision=prec)
+        e = max(np.abs(f1 - o1).max() / np.abs(o1).max(), np.abs(f2 - o2).max() / np.abs(o2).max())
+        print("L=%d Ls=%d: %s correlator error %.2e, iterations %d vs %d" % (L, Ls, prec, e, fit, it))
+        assert e <= 1e-6
     lat.close()
 
 
