@@ -1,0 +1,43 @@
+"""Twenty seeded RHMC trajectories at L=12, Ls=24 on the GPU against the CPU
+oracle's outputs for the same inputs (a fixture: the oracle needs about an
+hour for them)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+fm = pytest.importorskip("chiralheisenberg_amd.fermion")
+from chiralheisenberg_amd import remez, rhmc  # noqa: E402
+
+
+def test_trajectories_l12_ls24_match_oracle_fixture():
+    """Seeded trajectories at L=12, Ls=24 (beta 0.475, dt 0.05, mean length
+    20 steps, the reference's production tolerances 1e-9 / 1e-6) with every
+    fermion routine on the GPU: the same trajectory lengths and accept/reject
+    decisions as the CPU oracle's loop, H0 and H1 within 1e-9 relative, dH
+    within 1e-5, the chi and generator state after the last one.  The oracle
+    takes about a minute per molecular-dynamics step here, so its outputs are
+    the fixture tests/golden/oracle_traj_l12_ls24.npz (the first n of twenty
+    trajectories), written by tests/golden/make_oracle_traj_l12_ls24.py with
+    the same inputs."""
+    import os
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_oracle_traj_l12_ls24 as gen
+    ref = np.load(os.path.join(here, "golden", "oracle_traj_l12_ls24.npz"))
+    sets = {k: remez.load(k) for k in ("remez2", "remez4", "remez2g")}
+    lat = fm.Lattice(gen.L, gen.LS)
+    chi = gen.chi0()
+    rng = rhmc.RandomStream(gen.SEED)
+    n = len(ref["dH"])
+    gpu = [rhmc.trajectory(lat, chi, rng, isweep=i + 1, par=gen.PAR, sets=sets) for i in range(n)]
+    lat.close()
+    assert [g["md_steps"] for g in gpu] == list(ref["md_steps"])
+    assert [int(g["accepted"]) for g in gpu] == [int(a) for a in ref["accepted"]]
+    for i, g in enumerate(gpu):
+        for k in ("H0", "H1"):
+            assert abs(g[k] - ref[k][i]) <= 1e-9 * abs(ref[k][i]), (i, k, g[k], ref[k][i])
+        assert abs(g["dH"] - ref["dH"][i]) <= 1e-5, (i, g["dH"], ref["dH"][i])
+    assert np.abs(chi - ref["chi_final"]).max() <= 1e-6 * np.abs(ref["chi_final"]).max()
+    assert rng.seed() == float(ref["seed_final"])
