@@ -54,7 +54,8 @@ static thread_local std::string g_create_err;
 
 namespace {
 
-enum BufId { B_PHI = 0, B_X, B_R, B_Q0, B_Q1, B_V, B_W, B_CHI, B_F, B_PP, B_P, B_T1, B_T2, B_XS, B_RS, B_PS, B_STAGE };
+enum BufId { B_PHI = 0, B_X, B_R, B_Q0, B_Q1, B_V, B_W, B_CHI, B_F, B_PP, B_P, B_T1, B_T2, B_XS, B_RS, B_PS, B_PS2,
+             B_STAGE };
 
 int fail(chs_ctx *c, int code, const char *fmt, ...) {
     char buf[512];
@@ -927,12 +928,14 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
     const size_t bs = sizeof(typename S::st), ba = sizeof(typename SA::st);
     int rc;
     if ((rc = ensure_spinors(c, {B_T1, B_T2, B_R}))) return rc;   // FP64 scratch + true residual
-    if ((rc = ensure(c, c->b[B_PS], n * bs))) return rc;         // p (dslash input)
+    if ((rc = ensure(c, c->b[B_PS], n * bs))) return rc;         // p of even iterations (dslash input)
+    if ((rc = ensure(c, c->b[B_PS2], n * bs))) return rc;        // p of odd iterations
     if ((rc = ensure(c, c->b[B_V], n * bs))) return rc;          // M p
     if ((rc = ensure(c, c->b[B_XS], n * ba))) return rc;         // low-precision x accumulator
     if ((rc = ensure(c, c->b[B_RS], n * ba))) return rc;         // iterated residual
     double2 *t1 = buf<double2>(c, B_T1), *t2 = buf<double2>(c, B_T2), *r64 = buf<double2>(c, B_R);
     void *ps = c->b[B_PS].p, *mp = c->b[B_V].p, *xs = c->b[B_XS].p, *rs = c->b[B_RS].p;
+    void *const pbuf[2] = {ps, c->b[B_PS2].p};   // p_k lives in pbuf[k % 2]
     const double2 *b = (const double2 *)bvec;
     double2 *x = (double2 *)xvec;
     const unsigned gn = nblocks(n);
@@ -991,23 +994,35 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
         // (alpha sigma for scaled storage)
         const double *alpha_ptr = S::scaled ? &c->cg->alpha_s : &c->cg->alpha;
         const int *skip = &c->cg->halt;
-        const int chunk = chunk_for(c, 11.0 * bs * n);
-        // one chunk of iterations: five launches each, replayed from a
-        // hipGraph (the pointers are fixed for the whole solve)
+        // an even number of iterations per chunk: a chunk starts at an even
+        // iteration, so its p buffers are the same in every replay
+        const int chunk = std::max(2, chunk_for(c, 10.0 * bs * n) & ~1);
+        typedef typename S::st sst;
+        typedef typename SA::st ast;
+        // iteration k (parity par = k % 2): five launches
+        auto enqueue_iter = [&](int par) {
+            void *pc = pbuf[par], *po = pbuf[par ^ 1];
+            dslash_launch<S, SA, false, EPI_NORM>(c, mp, pc, chi, hpin, imass,
+                                                 epi(c->partial, nullptr, nullptr, nullptr, nullptr, skip));
+            hipLaunchKernelGGL(k_rcg_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->cg, (const double *)c->partial,
+                               np);
+            // r -= alpha M^dagger M p (the x increments are in the p update)
+            dslash_launch<S, SA, true, EPI_CG>(c, nullptr, mp, chi, hpin, imass,
+                                              epi(c->partial, alpha_ptr, pc, xs, rs, skip, 1));
+            hipLaunchKernelGGL(k_rcg_beta, dim3(1), dim3(kBlock), 0, c->stream, c->cg, (const double *)c->partial,
+                               np);
+            if (par == 0)
+                hipLaunchKernelGGL((k_cg_pnext<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, (const sst *)pc,
+                                   (sst *)po, (const ast *)rs, (const CgState *)c->cg);
+            else
+                hipLaunchKernelGGL((k_cg_xpnext<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, (const sst *)pc,
+                                   (sst *)po, (const ast *)rs, (ast *)xs, (const CgState *)c->cg);
+        };
+        // one chunk, replayed from a hipGraph (the pointers are fixed)
         auto enqueue_chunk = [&]() {
-            for (int k = 0; k < chunk; k++) {
-                dslash_launch<S, SA, false, EPI_NORM>(c, mp, ps, chi, hpin, imass,
-                                                     epi(c->partial, nullptr, nullptr, nullptr, nullptr, skip));
-                hipLaunchKernelGGL(k_rcg_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->cg,
-                                   (const double *)c->partial, np);
-                // r -= alpha M^dagger M p (the x increment moves to the next kernel)
-                dslash_launch<S, SA, true, EPI_CG>(c, nullptr, mp, chi, hpin, imass,
-                                                  epi(c->partial, alpha_ptr, ps, xs, rs, skip, 1));
-                hipLaunchKernelGGL(k_rcg_beta, dim3(1), dim3(kBlock), 0, c->stream, c->cg,
-                                   (const double *)c->partial, np);
-                hipLaunchKernelGGL((k_cg_xpupdate<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
-                                   (typename S::st *)ps, (const typename SA::st *)rs, (typename SA::st *)xs,
-                                   (const CgState *)c->cg);
+            for (int k = 0; k < chunk; k += 2) {
+                enqueue_iter(0);
+                enqueue_iter(1);
             }
         };
         GraphGuard gg;
@@ -1021,21 +1036,26 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
             HIPCHK(c, e);
         }
         for (;;) {
+            // after a reliable update at an even iteration the next one is
+            // odd: run it alone so that the chunk starts even again
+            if (h.iter & 1) enqueue_iter(1);
             if (gg.exec) HIPCHK(c, hipGraphLaunch(gg.exec, c->stream));
             else enqueue_chunk();
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipMemcpyAsync(&h, c->cg, sizeof h, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             if (!h.hold) continue;
-            // the increment of the step that raised the hold, then the
-            // reliable update: x += xs; r64 = b - M^dagger M x; rs = r64
-            hipLaunchKernelGGL((k_cg_xupdate<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
-                               (const typename S::st *)ps, (typename SA::st *)xs, (const CgState *)c->cg);
-            hipLaunchKernelGGL((k_accum_x<SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, x,
-                               (typename SA::st *)xs);
+            // iteration k raised the hold; its p update did not run.  Add the
+            // pending increments (a_k p_k, and for odd k a_{k-1} p_{k-1}), then
+            // the reliable update: x += xs; r64 = b - M^dagger M x; rs = r64
+            const int kh = h.iter - 1, par = kh & 1;
+            hipLaunchKernelGGL((k_cg_xflush<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
+                               (const sst *)pbuf[par], (const sst *)pbuf[par ^ 1], (ast *)xs,
+                               (const CgState *)c->cg, par);
+            hipLaunchKernelGGL((k_accum_x<SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, x, (ast *)xs);
             mdagm64(c, t2, x, t1, chi, hpin, imass);
             hipLaunchKernelGGL((k_true_resid<SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, b, (const double2 *)t2,
-                               r64, (typename SA::st *)rs, c->partial2);
+                               r64, (ast *)rs, c->partial2);
             hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBlock), 0, c->stream, (const double *)c->partial2,
                                (int)gn, c->dscal + 1);
             HIPCHK(c, hipMemcpyAsync(&rn2, c->dscal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1057,7 +1077,8 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
                 stop_fac *= 0.25;
                 h.stop2 = stop_fac * target2;
             }
-            // continue CG from the true residual with the search direction kept
+            // continue CG from the true residual with the search direction
+            // kept: p_{k+1} from p_k into the other buffer; nothing is pending
             h.beta = rn2 / h.rnorm2;
             if constexpr (S::scaled) {
                 const double sig_new = std::sqrt(rn2 / (double)n);
@@ -1065,13 +1086,14 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
                 h.xp_p = h.beta * h.sigma / sig_new;
                 h.sigma = sig_new;
             }
+            h.xp_a = 0.0;
             h.rnorm2 = rn2;
             h.maxr2 = rn2;
             h.hold = 0;
             h.halt = 0;
             HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
-            hipLaunchKernelGGL((k_cg_pupdate<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
-                               (typename S::st *)ps, (const typename SA::st *)rs, (const CgState *)c->cg);
+            hipLaunchKernelGGL((k_cg_pnext<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, (const sst *)pbuf[par],
+                               (sst *)pbuf[par ^ 1], (const ast *)rs, (const CgState *)c->cg);
         }
         st.iterations = h.iter;
     }

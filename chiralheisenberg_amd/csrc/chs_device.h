@@ -625,6 +625,10 @@ struct CgState {
     // stored M^dagger M p, and the x/p update is x += xp_a p, p = xp_r r +
     // xp_p p (xp_a = alpha sigma, xp_r = 1 / sigma', xp_p = beta sigma / sigma')
     double sigma, alpha_s, xp_a, xp_r, xp_p, nelem;
+    // the x increment coefficient of the previous iteration (alpha, or alpha
+    // sigma when scaled): the reliable-update CG adds two increments every
+    // second iteration (k_cg_xpnext)
+    double xp_a_prev;
     double resid;        // reference rule: stop when |r|^2 < resid
     double rnorm2;       // |r|^2 of the current iterated residual
     double maxr2;        // |r|^2 at the last reliable update
@@ -736,6 +740,78 @@ __global__ __launch_bounds__(kBlock) void k_cg_xupdate(long n, const typename S:
     }
 }
 
+// Reliable-update CG with two search-direction buffers used alternately
+// (p_k in buffer k mod 2): iteration k forms p_{k+1} = r + beta p_k (scaled:
+// r / sigma' + beta sigma / sigma' p_k) in the other buffer.  Even k defers
+// its x increment (k_cg_pnext); odd k adds both, x += a_{k-1} p_{k-1} +
+// a_k p_k, before p_{k+1} replaces p_{k-1} (k_cg_xpnext).  The accumulated x
+// is read and written every second iteration instead of every one.
+template <class S, class SA>
+__device__ __forceinline__ cx<typename S::rt> cg_pnew(cx<typename SA::rt> rv, cx<typename S::rt> pv,
+                                                      const CgState *__restrict__ st) {
+    typedef typename S::rt T;
+    if constexpr (S::scaled) {
+        const T cr = (T)st->xp_r, cp = (T)st->xp_p;
+        return cx<T>{cr * (T)rv.r + cp * pv.r, cr * (T)rv.i + cp * pv.i};
+    } else {
+        const T b = (T)st->beta;
+        return cx<T>{(T)rv.r + b * pv.r, (T)rv.i + b * pv.i};
+    }
+}
+
+template <class S, class SA>
+__global__ __launch_bounds__(kBlock) void k_cg_pnext(long n, const typename S::st *__restrict__ pc,
+                                                     typename S::st *__restrict__ po,
+                                                     const typename SA::st *__restrict__ r,
+                                                     const CgState *__restrict__ st) {
+    if (st->halt) return;
+    const long i = logical_block() * kBlock + threadIdx.x;
+    if (i < n) S::sto(po, i, cg_pnew<S, SA>(SA::ld(r, i), S::ld(pc, i), st));
+}
+
+template <class S, class SA>
+__global__ __launch_bounds__(kBlock) void k_cg_xpnext(long n, const typename S::st *__restrict__ pc,
+                                                      typename S::st *__restrict__ po,
+                                                      const typename SA::st *__restrict__ r,
+                                                      typename SA::st *__restrict__ x,
+                                                      const CgState *__restrict__ st) {
+    if (st->halt) return;
+    const long i = logical_block() * kBlock + threadIdx.x;
+    typedef typename SA::rt TA;
+    if (i < n) {
+        const TA a0 = (TA)st->xp_a_prev, a1 = (TA)st->xp_a;
+        const cx<typename S::rt> pv = S::ld(pc, i), pvo = S::ld(po, i);
+        cx<TA> xv = SA::ld(x, i);
+        xv = {xv.r + a0 * (TA)pvo.r, xv.i + a0 * (TA)pvo.i};
+        xv = {xv.r + a1 * (TA)pv.r, xv.i + a1 * (TA)pv.i};
+        SA::sto(x, i, xv);
+        S::sto(po, i, cg_pnew<S, SA>(SA::ld(r, i), pv, st));
+    }
+}
+
+// The increments still pending when the solve stops for the host after
+// iteration k: a_k p_k (alpha of that step), and for odd k also a_{k-1}
+// p_{k-1} (po) -- the update of that step did not run.
+template <class S, class SA>
+__global__ __launch_bounds__(kBlock) void k_cg_xflush(long n, const typename S::st *__restrict__ pc,
+                                                      const typename S::st *__restrict__ po,
+                                                      typename SA::st *__restrict__ x,
+                                                      const CgState *__restrict__ st, int both) {
+    const long i = logical_block() * kBlock + threadIdx.x;
+    typedef typename SA::rt TA;
+    if (i < n) {
+        const TA a1 = (TA)(S::scaled ? st->alpha_s : st->alpha);
+        cx<TA> xv = SA::ld(x, i);
+        if (both) {
+            const TA a0 = (TA)st->xp_a;
+            const cx<typename S::rt> pvo = S::ld(po, i);
+            xv = {xv.r + a0 * (TA)pvo.r, xv.i + a0 * (TA)pvo.i};
+        }
+        const cx<typename S::rt> pv = S::ld(pc, i);
+        SA::sto(x, i, cx<TA>{xv.r + a1 * (TA)pv.r, xv.i + a1 * (TA)pv.i});
+    }
+}
+
 // Reliable-update CG scalars.  alpha = rnorm2 / |M p|^2; after |r_s|^2 of
 // the iterated residual either a plain step (beta) or a hold for the host
 // (reliable update due, converged candidate, iteration cap, or a non-finite
@@ -760,7 +836,7 @@ __global__ void k_rcg_alpha(CgState *st, const double *__restrict__ partial, int
 __global__ void k_rcg_beta(CgState *st, const double *__restrict__ partial, int np) {
     const int halt = st->halt, iter = st->iter + 1, maxiter = st->maxiter, scaled = st->scaled;
     const double rn_old = st->rnorm2, d2 = st->delta2, maxr2 = st->maxr2, stop2 = st->stop2;
-    const double sig = st->sigma, alpha_s = st->alpha_s, nelem = st->nelem;
+    const double sig = st->sigma, alpha_s = st->alpha_s, nelem = st->nelem, xpa = st->xp_a, alpha = st->alpha;
     const double rn = sum_partials(partial, np);
     if (threadIdx.x != 0 || halt) return;
     st->iter = iter;
@@ -772,12 +848,15 @@ __global__ void k_rcg_beta(CgState *st, const double *__restrict__ partial, int 
         const double beta = rn / rn_old;
         st->beta = beta;
         st->rnorm2 = rn;
+        st->xp_a_prev = xpa;
         if (scaled) {
             const double sig_new = sqrt(rn / nelem);
             st->xp_a = alpha_s;
             st->xp_r = 1.0 / sig_new;
             st->xp_p = beta * sig / sig_new;
             st->sigma = sig_new;
+        } else {
+            st->xp_a = alpha;
         }
     }
 }
