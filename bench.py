@@ -282,8 +282,8 @@ def main():
                   flush=True)
 
     # rooflines: the FP64 dslash (2 V64 + C), the kernel the metric names, and
-    # the dominant kernel of the timed FP32 solve, its fused CG stencil
-    # (M^dagger M p with r -= alpha M^dagger M p and |r|^2; 3 V32 + C).
+    # the dominant kernel of the timed solve, its fused CG stencil (M^dagger
+    # M p with r -= alpha M^dagger M p and |r|^2).
     kvol = lat.kvol
     V = lat.kferm * 16
     V32 = lat.kferm * 8
