@@ -144,6 +144,25 @@ def test_solve_mdagm_deterministic(prec, monkeypatch):
     assert np.array_equal(xs[0], xs[1]) and np.array_equal(xs[0], xs[2])
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_solve_mdagm_large_lattice(prec):
+    """L=32, Ls=20: 5120 workgroup partials per stencil, more than the 4096
+    the reductions take in their first batch of loads, and a 5.2M-element
+    spinor; the FP64 true residual is checked with the oracle's operator."""
+    lat = fm.Lattice(32, 20)
+    olat = orc.Lattice(32, 20)
+    rs = np.random.RandomState(74)
+    b, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    bd = fm.spinor_dev(lat, b)
+    xd = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+    st = fm.solve_mdagm(lat, bd, xd, fm.DeviceArray.from_numpy(lat, chi), 1e-10, prec)
+    x = fm.to_host(lat, xd)
+    lat.close()
+    r = b - orc.dslash(olat, orc.dslash(olat, x, chi, 0.0, 1, False), chi, 0.0, 1, True)
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    assert rel <= 1.05e-10, (prec, rel, st)
+
+
 @pytest.mark.parametrize("scale", [1e-20, 1e20])
 def test_solve_mdagm_fp16_scaled_source(scale):
     """The binary16 form stores p and M p divided by the residual's rms: a
