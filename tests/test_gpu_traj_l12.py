@@ -41,3 +41,29 @@ def test_trajectories_l12_ls24_match_oracle_fixture():
         assert abs(g["dH"] - ref["dH"][i]) <= 1e-5, (i, g["dH"], ref["dH"][i])
     assert np.abs(chi - ref["chi_final"]).max() <= 1e-6 * np.abs(ref["chi_final"]).max()
     assert rng.seed() == float(ref["seed_final"])
+
+
+def test_trajectories_l12_ls24_fp32_force_same_decisions():
+    """The same trajectories with the force's multi-shift solves in FP32
+    storage (Lanczos scalars, shift sums and derivs in FP64; hamilton and the
+    Metropolis test FP64): the same lengths and accept/reject decisions as the
+    all-FP64 CPU oracle, dH within 1e-3 of it."""
+    import os
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_oracle_traj_l12_ls24 as gen
+    ref = np.load(os.path.join(here, "golden", "oracle_traj_l12_ls24.npz"))
+    sets = {k: remez.load(k) for k in ("remez2", "remez4", "remez2g")}
+    lat = fm.Lattice(gen.L, gen.LS)
+    fm.set_force_precision(lat, "fp32")
+    chi = gen.chi0()
+    rng = rhmc.RandomStream(gen.SEED)
+    n = len(ref["dH"])
+    gpu = [rhmc.trajectory(lat, chi, rng, isweep=i + 1, par=gen.PAR, sets=sets) for i in range(n)]
+    lat.close()
+    dh = [abs(g["dH"] - ref["dH"][i]) for i, g in enumerate(gpu)]
+    print("FP32 force: max |dH - dH_oracle| = %.2e over %d trajectories" % (max(dh), n))
+    assert [g["md_steps"] for g in gpu] == list(ref["md_steps"])
+    assert [int(g["accepted"]) for g in gpu] == [int(a) for a in ref["accepted"]]
+    assert max(dh) <= 1e-3
