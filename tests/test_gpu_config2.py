@@ -1,0 +1,66 @@
+"""Single operations at L=Lt=12, Ls=24 (the north star's second
+configuration: beta 0.475, imass 1, seeded Gaussian phi_a and spinors) on the
+GPU against the CPU oracle's results for the same inputs, kept as the fixture
+tests/golden/oracle_config2_l12_ls24.npz (written by
+make_oracle_config2_l12_ls24.py: the oracle needs minutes for them):
+dslash / dslashd within 1e-13, the remez4 multi-shift solution at res 1e-10
+within 1e-9 with the iteration count within 5, the force within 1e-10 and
+the Hamiltonian within 1e-10."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+fm = pytest.importorskip("chiralheisenberg_amd.fermion")
+from chiralheisenberg_amd import remez  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import make_oracle_config2_l12_ls24 as gen  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def setup():
+    ref = np.load(os.path.join(HERE, "golden", "oracle_config2_l12_ls24.npz"))
+    lat = fm.Lattice(gen.L, gen.LS)
+    yield lat, ref, gen.inputs()
+    lat.close()
+
+
+def test_dslash_config2(setup):
+    lat, ref, (chi, R, Phi, pp, idx) = setup
+    for dag in (0, 1):
+        y = (fm.dslashd if dag else fm.dslash)(lat, R, chi, 0.0, 0.5, 1).reshape(-1)
+        s = ref["dslash_d%d_s" % dag]
+        assert np.abs(y[idx] - s).max() <= 1e-13 * np.abs(s).max()
+        n = float(ref["dslash_d%d_n" % dag])
+        assert abs(np.linalg.norm(y) - n) <= 1e-13 * n
+
+
+def test_qmrherm_remez4_config2(setup):
+    lat, ref, (chi, R, Phi, pp, idx) = setup
+    a4, ad4 = remez.load("remez4")[:2]
+    x, it = fm.qmrherm(lat, Phi, 1e-10, a4, ad4, chi, 0.0, 0.0, 1, 0)
+    x = x.reshape(-1)
+    s = ref["qmr_s"]
+    assert abs(it - int(ref["qmr_it"])) <= 5, (it, int(ref["qmr_it"]))
+    assert np.abs(x[idx] - s).max() <= 1e-9 * np.abs(s).max()
+    n = float(ref["qmr_n"])
+    assert abs(np.linalg.norm(x) - n) <= 1e-9 * n
+
+
+def test_force_config2(setup):
+    lat, ref, (chi, R, Phi, pp, idx) = setup
+    F, _, _ = fm.force(lat, Phi, 1e-12, chi, gen.BETA, remez.load("remez2g"), 0.0, 0.0, 1)
+    Fc = ref["force"]
+    assert np.abs(F - Fc).max() <= 1e-10 * np.abs(Fc).max()
+
+
+def test_hamilton_config2(setup):
+    lat, ref, (chi, R, Phi, pp, idx) = setup
+    h, _, _ = fm.hamilton(lat, Phi, 1e-11, chi, pp, gen.BETA, remez.load("remez2"))
+    for a, b in zip(h, ref["ham"]):
+        assert abs(a - b) <= 1e-10 * abs(b), (h, ref["ham"])
