@@ -146,6 +146,47 @@ def batch_bench(mix="full", npf=4, res=1e-9, seed=11, precision="fp64"):
     return out
 
 
+def sweep_bench(tol=1e-10, Ls=32):
+    """Every solve form at L in {16, 24}, Ls=32 and beta in {0.45, 0.475,
+    0.5}: iterations, reliable updates, device time to the FP64 true relative
+    residual tol, and each mixed form's solution error against the FP64 GPU
+    solve of the same system (relative 2-norm)."""
+    from chiralheisenberg_amd import fermion as fm
+    rows = []
+    for L in (16, 24):
+        lat = fm.Lattice(L, Ls)
+        for beta in (0.45, 0.475, 0.5):
+            chi, b = make_inputs(lat, beta)
+            chi_d = fm.DeviceArray.from_numpy(lat, chi)
+            b_d = fm.spinor_dev(lat, b)
+            x_d = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+            sol = {}
+            for form in ("fp64", "fp32", "fp16", "bf16"):
+                try:
+                    x_d.upload(np.zeros(lat.shape, np.complex128))
+                    fm.solve_mdagm(lat, b_d, x_d, chi_d, tol, form)   # warm-up (allocations, graph)
+                    x_d.upload(np.zeros(lat.shape, np.complex128))
+                    st = fm.solve_mdagm(lat, b_d, x_d, chi_d, tol, form)
+                except fm._lib.NoConvergence as e:
+                    rows.append({"L": L, "Ls": Ls, "beta": beta, "form": form, "converged": False, "error": str(e)})
+                    print("bench: sweep L=%d beta=%g %s: %s" % (L, beta, form, e), file=sys.stderr, flush=True)
+                    continue
+                sol[form] = fm.to_host(lat, x_d)
+                err = np.linalg.norm(sol[form] - sol["fp64"]) / np.linalg.norm(sol["fp64"])
+                rows.append({"L": L, "Ls": Ls, "beta": beta, "form": form, "converged": True,
+                             "iterations": st["iterations"],
+                             "reliable_updates": st["reliable_updates"], "ms": st["seconds"] * 1e3,
+                             "true_resid": st["true_resid"], "solution_error_vs_fp64": float(err)})
+                print("bench: sweep L=%d beta=%g %s: %d its, %d updates, %.1f ms, resid %.2e, error %.2e" % (
+                    L, beta, form, st["iterations"], st["reliable_updates"], st["seconds"] * 1e3,
+                    st["true_resid"], err), file=sys.stderr, flush=True)
+            for a in (chi_d, b_d, x_d):
+                a.free()
+        lat.close()
+    return {"workload": "M^dagger M solves to FP64 true relative residual %g, L in {16, 24}, Ls=%d, beta in "
+                        "{0.45, 0.475, 0.5}, every storage form" % (tol, Ls), "rows": rows}
+
+
 def learned_bench(fm, lat, chi_d, b_d, tol, npairs=8, epochs=10, lr=2e-3):
     """The learned preconditioner on the bench system: training pairs from the
     plain FP32 solver, Adam training on the GPU, then flexible CG (FP32, FP64
@@ -214,6 +255,8 @@ def main():
     ap.add_argument("--counters", action="store_true",
                     help="only the roofline kernels and calibration copies (for the counter runs)")
     ap.add_argument("--batch", action="store_true", help="only the batched multi-shift workload (one JSON line)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="only the solve forms at L in {16, 24}, Ls=32, beta in {0.45, 0.475, 0.5} (one JSON line)")
     ap.add_argument("--no-extras", action="store_true", help="skip the learned-preconditioner and force figures")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -221,6 +264,9 @@ def main():
               "shards by independent lattice/chain, so --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.sweep:
+        print(json.dumps(sweep_bench(args.tol, args.Ls)), flush=True)
+        return
     if args.batch:
         print(json.dumps({"%s_%s" % (m, p): batch_bench(m, precision=p) for m in BATCH_MIXES
                           for p in ("fp64", "fp32")}), flush=True)

@@ -1025,6 +1025,13 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
                 enqueue_iter(1);
             }
         };
+        // a solve whose true residual has not improved on its best in kStall
+        // reliable updates in a row is diverging (BF16 storage does, at L=16,
+        // Ls=32, beta=0.45): it stops with CHS_ENOCONV instead of running to
+        // maxiter.  Converging forms gain ~1/delta^2 in |r|^2 per update.
+        constexpr int kStall = 8;
+        double best_rn2 = rn2;
+        int stall = 0;
         GraphGuard gg;
         if (c->use_graphs) {
             hipGraph_t graph;
@@ -1058,8 +1065,20 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
                                r64, (ast *)rs, c->partial2);
             hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBlock), 0, c->stream, (const double *)c->partial2,
                                (int)gn, c->dscal + 1);
-            HIPCHK(c, hipMemcpyAsync(&rn2, c->dscal + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            // Re <r, p_k>: the kept direction's component along the new residual
+            {
+                long nbd = std::min<long>(2048, (n + kBlock - 1) / kBlock);
+                nbd = (nbd + 7) / 8 * 8;
+                hipLaunchKernelGGL((k_dot_rp<S>), dim3((unsigned)nbd), dim3(kBlock), 0, c->stream, n,
+                                   (const double2 *)r64, (const sst *)pbuf[par], c->partial2);
+                hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(kBlock), 0, c->stream, (const double *)c->partial2,
+                                   (int)nbd, c->dscal + 2);
+            }
+            double rd[2];
+            HIPCHK(c, hipMemcpyAsync(rd, c->dscal + 1, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
+            rn2 = rd[0];
+            const double rp = rd[1];
             st.reliable_updates++;
             if (rn2 <= target2) break;
             if (!(rn2 == rn2)) {
@@ -1071,20 +1090,36 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
                               std::sqrt(rn2 / bnorm2));
                 break;
             }
+            if (rn2 < best_rn2) {
+                best_rn2 = rn2;
+                stall = 0;
+            } else if (++stall >= kStall) {
+                status = fail(c, CHS_ENOCONV,
+                              "true residual diverging: %g after %d iterations, best %g, no improvement in %d "
+                              "reliable updates",
+                              std::sqrt(rn2 / bnorm2), h.iter, std::sqrt(best_rn2 / bnorm2), kStall);
+                break;
+            }
             // the iterated residual had reached the stopping region but the
             // true one had not: tighten the trigger
             if (h.rn_hold < h.stop2) {
                 stop_fac *= 0.25;
                 h.stop2 = stop_fac * target2;
             }
-            // continue CG from the true residual with the search direction
-            // kept: p_{k+1} from p_k into the other buffer; nothing is pending
+            // continue CG from the true residual with the search direction kept,
+            // first made orthogonal to the new residual (p -= (<r,p>/|r|^2) r,
+            // as the reliable-update CG of QUDA does): p_{k+1} = r + beta p_k'
+            // from p_k into the other buffer; nothing is pending
             h.beta = rn2 / h.rnorm2;
+            const double cr = rp / rn2;   // in p's stored units
             if constexpr (S::scaled) {
                 const double sig_new = std::sqrt(rn2 / (double)n);
-                h.xp_r = 1.0 / sig_new;
                 h.xp_p = h.beta * h.sigma / sig_new;
+                h.xp_r = 1.0 / sig_new - h.xp_p * cr;
                 h.sigma = sig_new;
+            } else {
+                h.xp_p = h.beta;
+                h.xp_r = 1.0 - h.beta * cr;
             }
             h.xp_a = 0.0;
             h.rnorm2 = rn2;
@@ -1092,8 +1127,8 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
             h.hold = 0;
             h.halt = 0;
             HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
-            hipLaunchKernelGGL((k_cg_pnext<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n, (const sst *)pbuf[par],
-                               (sst *)pbuf[par ^ 1], (const ast *)rs, (const CgState *)c->cg);
+            hipLaunchKernelGGL((k_cg_prestart<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
+                               (const sst *)pbuf[par], (sst *)pbuf[par ^ 1], (const ast *)rs, (const CgState *)c->cg);
         }
         st.iterations = h.iter;
     }

@@ -789,6 +789,42 @@ __global__ __launch_bounds__(kBlock) void k_cg_xpnext(long n, const typename S::
     }
 }
 
+// Re <r, p> over n complex numbers, r FP64 and p in storage S, one partial per
+// workgroup (grid-stride): the component of the kept search direction along
+// the new true residual at a reliable update.
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_dot_rp(long n, const double2 *__restrict__ r,
+                                                   const typename S::st *__restrict__ p,
+                                                   double *__restrict__ partial) {
+    const long lb = logical_block();
+    const long stride = (long)gridDim.x * kBlock;
+    double v = 0.0;
+    for (long i = lb * kBlock + threadIdx.x; i < n; i += stride) {
+        const double2 rv = r[i];
+        const cx<typename S::rt> pv = S::ld(p, i);
+        v += rv.x * (double)pv.r + rv.y * (double)pv.i;
+    }
+    const double t = block_sum(v);
+    if (threadIdx.x == 0) partial[lb] = t;
+}
+
+// The direction after a reliable update: p_out = xp_r r + xp_p p_in (the host
+// folds the removal of p's component along the new residual into xp_r).
+template <class S, class SA>
+__global__ __launch_bounds__(kBlock) void k_cg_prestart(long n, const typename S::st *__restrict__ pc,
+                                                        typename S::st *__restrict__ po,
+                                                        const typename SA::st *__restrict__ r,
+                                                        const CgState *__restrict__ st) {
+    const long i = logical_block() * kBlock + threadIdx.x;
+    typedef typename S::rt T;
+    if (i < n) {
+        const T a = (T)st->xp_r, b = (T)st->xp_p;
+        const cx<typename SA::rt> rv = SA::ld(r, i);
+        const cx<T> pv = S::ld(pc, i);
+        S::sto(po, i, cx<T>{a * (T)rv.r + b * pv.r, a * (T)rv.i + b * pv.i});
+    }
+}
+
 // The increments still pending when the solve stops for the host after
 // iteration k: a_k p_k (alpha of that step), and for odd k also a_{k-1}
 // p_{k-1} (po) -- the update of that step did not run.
@@ -840,7 +876,10 @@ __global__ void k_rcg_beta(CgState *st, const double *__restrict__ partial, int 
     const double rn = sum_partials(partial, np);
     if (threadIdx.x != 0 || halt) return;
     st->iter = iter;
-    if (rn < d2 * maxr2 || rn < stop2 || iter >= maxiter || !(rn == rn)) {
+    // hold for the host: reliable update due, converged candidate, iteration
+    // cap, a non-finite value, or an iterated |r| 1000x above the last true
+    // one (a diverging solve, which the host's stall test then ends)
+    if (rn < d2 * maxr2 || rn < stop2 || iter >= maxiter || !(rn == rn) || rn > 1e6 * maxr2) {
         st->hold = 1;
         st->halt = 1;
         st->rn_hold = rn;
