@@ -212,7 +212,11 @@ typedef struct {
  * CG with FP64 true residuals every time the inner residual has dropped by
  * delta, delta <= 0 picks a default; in CHS_FP16 the search direction and its
  * image are stored in binary16 divided by the residual's rms, the residual
- * and the x increment in FP32).  x holds the initial guess on entry. */
+ * and the x increment in FP32).  x holds the initial guess on entry.
+ * Returns CHS_ENOCONV after maxiter iterations (maxiter <= 0: 100000), on a
+ * non-finite residual, or when the FP64 true residual has not improved in 8
+ * reliable updates in a row (a diverging low-precision solve: BF16 storage
+ * at L=16, Ls=32, beta=0.45 is one). */
 int chs_solve_mdagm_dev(chs_ctx *ctx, const double *b, double *x,
                         const double *chi, double hpin, int imass, double tol,
                         int prec, double delta, int maxiter,
