@@ -692,54 +692,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_pupdate(long n, typename S::st *_
     }
 }
 
-// x += alpha p and p = r + beta p in one pass: the x increment of the step
-// that produced r and the next direction both use the old p (so the fused CG
-// stencil no longer reads p and x: one spinor fewer per iteration)
-template <class S, class SA>
-__global__ __launch_bounds__(kBlock) void k_cg_xpupdate(long n, typename S::st *__restrict__ p,
-                                                        const typename SA::st *__restrict__ r,
-                                                        typename SA::st *__restrict__ x,
-                                                        const CgState *__restrict__ st) {
-    if (st->halt) return;
-    const long i = logical_block() * kBlock + threadIdx.x;
-    typedef typename S::rt T;
-    typedef typename SA::rt TA;
-    if (i < n) {
-        const cx<TA> rv = SA::ld(r, i);
-        const cx<T> pv = S::ld(p, i);
-        cx<TA> xv = SA::ld(x, i);
-        if constexpr (S::scaled) {
-            const TA a = (TA)st->xp_a;
-            const T cr = (T)st->xp_r, cp = (T)st->xp_p;
-            xv = {xv.r + a * (TA)pv.r, xv.i + a * (TA)pv.i};
-            SA::sto(x, i, xv);
-            S::sto(p, i, cx<T>{cr * (T)rv.r + cp * pv.r, cr * (T)rv.i + cp * pv.i});
-        } else {
-            const T b = (T)st->beta;
-            const TA a = (TA)st->alpha;
-            xv = {xv.r + a * (TA)pv.r, xv.i + a * (TA)pv.i};
-            SA::sto(x, i, xv);
-            S::sto(p, i, cx<T>{(T)rv.r + b * pv.r, (T)rv.i + b * pv.i});
-        }
-    }
-}
-
-// x += alpha p alone: the last step's increment when the solve stopped for
-// the host (the fused update above did not run)
-template <class S, class SA>
-__global__ __launch_bounds__(kBlock) void k_cg_xupdate(long n, const typename S::st *__restrict__ p,
-                                                       typename SA::st *__restrict__ x,
-                                                       const CgState *__restrict__ st) {
-    const long i = logical_block() * kBlock + threadIdx.x;
-    typedef typename SA::rt TA;
-    if (i < n) {
-        const TA a = (TA)(S::scaled ? st->alpha_s : st->alpha);
-        const cx<typename S::rt> pv = S::ld(p, i);
-        cx<TA> xv = SA::ld(x, i);
-        SA::sto(x, i, cx<TA>{xv.r + a * (TA)pv.r, xv.i + a * (TA)pv.i});
-    }
-}
-
 // Reliable-update CG with two search-direction buffers used alternately
 // (p_k in buffer k mod 2): iteration k forms p_{k+1} = r + beta p_k (scaled:
 // r / sigma' + beta sigma / sigma' p_k) in the other buffer.  Even k defers
