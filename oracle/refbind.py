@@ -100,11 +100,21 @@ class Ref:
     def _r(self, v):
         return ctypes.byref(self.real(v))
 
-    def dslash(self, r, chi, hpin=0.0, imass=1, dagger=False, am=0.0):
+    def init_tables(self):
+        """init(nc) with nc < 0 (rhmc/cheisdwf.f:1491-1597): neighbour tables
+        and the gamma tables /dirac/ that dslash_v1 reads; chi untouched."""
+        self.lib.init_(ctypes.byref(ctypes.c_int(-1)))
+
+    def dslash(self, r, chi, hpin=0.0, imass=1, dagger=False, am=0.0, v1=False):
+        """dslash / dslashd (rhmc/cheisdwf.f:2634-3501), or with v1=True the
+        legacy dslash_v1 / dslashd_v1 (:1804-2632; call init_tables first)."""
         r = np.ascontiguousarray(r, dtype=np.complex128)
         chi = np.ascontiguousarray(chi, dtype=np.float64)
         out = np.empty_like(r)
-        fn = self.lib.dslashd_ if dagger else self.lib.dslash_
+        if v1:
+            fn = self.lib.dslashd_v1_ if dagger else self.lib.dslash_v1_
+        else:
+            fn = self.lib.dslashd_ if dagger else self.lib.dslash_
         fn(out.ctypes.data_as(ctypes.c_void_p), r.ctypes.data_as(ctypes.c_void_p),
            chi.ctypes.data_as(ctypes.c_void_p), self._r(am), self._r(hpin), ctypes.byref(ctypes.c_int(imass)))
         return out

@@ -22,7 +22,10 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, ROOT)
 
 RNG_SEED = 693764523745.0
-CASES = [dict(imass=1, hpin=0.0, res=1e-11), dict(imass=3, hpin=0.5, res=1e-8)]
+# the third case runs with ibound = +1: periodic in t, and the propagator's
+# wrap-around sign isign = ibound (rhmc/cheisdwf.f:1021) is +1
+CASES = [dict(imass=1, hpin=0.0, res=1e-11), dict(imass=3, hpin=0.5, res=1e-8),
+         dict(imass=1, hpin=0.3, res=1e-11, ibound=1)]
 
 
 def run_case(case, out_json):
@@ -30,7 +33,7 @@ def run_case(case, out_json):
     from chiralheisenberg_amd import checkpoint
     chi, _, _ = checkpoint.read_con(os.path.join(HERE, "con_l12_fp64"), kvol=RB.KVOL)
     ref = RB.Ref("fp64")
-    ref.set_params(0.45, 1.0, -1)
+    ref.set_params(0.45, 1.0, case.get("ibound", -1))
     # the main stream after one draw, as the driver leaves it before a
     # measurement (the measurement draws the next number)
     ref.rranset(RNG_SEED)
@@ -55,6 +58,7 @@ def main():
         out["case%d_imass" % k] = case["imass"]
         out["case%d_hpin" % k] = case["hpin"]
         out["case%d_res" % k] = case["res"]
+        out["case%d_ibound" % k] = case.get("ibound", -1)
         out["case%d_cferm1" % k] = np.array(r["c1r"]) + 1j * np.array(r["c1i"])
         out["case%d_cferm2" % k] = np.array(r["c2r"]) + 1j * np.array(r["c2i"])
         out["case%d_aviter" % k] = r["aviter"]

@@ -20,10 +20,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def test_fermion_vs_reference_golden():
+    """imass 1 and 3 with the antiperiodic t boundary, and imass 1 periodic
+    (ibound = +1)."""
+    from test_oracle import fermion_cases
     g = np.load(os.path.join(HERE, "golden", "ref_fermion_l12_ls8.npz"))
     chi, _, _ = checkpoint.read_con(os.path.join(HERE, "golden", "con_l12_fp64"), kvol=1728)
-    lat = fm.Lattice(12, 8)
-    for k, tol in ((0, 5e-9), (1, 1e-5)):
+    cases = list(fermion_cases(g))
+    assert len(cases) == 3
+    for k, ibound, tol in cases:
+        lat = fm.Lattice(12, 8, ibound=ibound)
         rng = orc.Rng(float(g["rng_seed"]))   # the reference generator, restated
         rng.rano(0)
         ittt0 = int(12 * rng.rano(0)) + 1
@@ -36,7 +41,7 @@ def test_fermion_vs_reference_golden():
                                                                           float(g["case%d_aviter" % k])))
         assert e1 <= tol and e2 <= tol
         assert abs(it / 10 - float(g["case%d_aviter" % k])) <= 3
-    lat.close()
+        lat.close()
 
 
 @pytest.mark.parametrize("L,Ls,Lt,imass,ittt0", [(6, 4, 10, 1, 3), (8, 6, 8, 3, 8)])

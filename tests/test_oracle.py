@@ -118,6 +118,36 @@ def test_dslash_vs_reference_build_direct():
             assert np.abs(a - b).max() <= 1e-14 * np.abs(b).max()
 
 
+@pytest.mark.skipif(not RB.available("fp64"), reason="reference build oracle/_ref absent")
+@pytest.mark.parametrize("ibound", [-1, 1])
+def test_dslash_vs_both_reference_formulations(ibound):
+    """The reference holds two formulations of M and M^dagger: dslash/dslashd
+    (rhmc/cheisdwf.f:2634-3501) and the legacy dslash_v1/dslashd_v1
+    (:1804-2632, gamma tables from init, :1491-1597).  The oracle agrees with
+    both, with the program's antiperiodic t boundary (ibound = -1, :85) and
+    with the periodic one the routines apply for any other ibound (:1893,
+    :2831)."""
+    ref = RB.Ref("fp64")
+    ref.init_tables()
+    lat = orc.Lattice(12, 8, ibound=ibound)
+    rs = np.random.RandomState(4 + ibound)
+    R, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    try:
+        ref.set_params(0.45, 1.0, ibound)
+        for imass in (1, 3):
+            for dag in (False, True):
+                a = orc.dslash(lat, R, chi, 0.7, imass, dag)
+                for v1 in (False, True):
+                    b = ref.dslash(R, chi, 0.7, imass, dag, v1=v1)
+                    assert np.abs(a - b).max() <= 1e-14 * np.abs(b).max(), (imass, dag, v1)
+        if ibound == 1:
+            # the boundary matters: the antiperiodic oracle differs
+            a = orc.dslash(LAT12, R, chi, 0.7, 1, False)
+            assert np.abs(a - ref.dslash(R, chi, 0.7, 1, False)).max() > 0.1
+    finally:
+        ref.set_params(0.45, 1.0, -1)
+
+
 @pytest.mark.skipif(not RB.available("shipped"), reason="reference build oracle/_ref absent")
 def test_shipped_semantics_is_fp32_chi():
     """The shipped build rounds chi to FP32 inside dslash
@@ -339,16 +369,29 @@ def test_remez_sets_approximate_powers(name, p, tol):
     assert np.max(np.abs(rb * x ** p - 1)) <= tol
 
 
+def fermion_cases(g):
+    """(case index, ibound, correlator tolerance) of the fermion fixture: the
+    solver's stopping residual sets the tolerance (5e-9 at res 1e-11, 1e-5 at
+    1e-8)."""
+    k = 0
+    while "case%d_res" % k in g.files:
+        yield k, int(g["case%d_ibound" % k]), (5e-9 if float(g["case%d_res" % k]) < 1e-9 else 1e-5)
+        k += 1
+
+
 def test_fermion_vs_reference_golden():
     """orc_fermion against the reference's own fermion routine (flang build,
     run by tests/golden/make_golden_fermion.py) on the reference's thermalised
     configuration rhmc/_con: correlators to the solver tolerance, iteration
-    counts within a few per solve."""
+    counts within a few per solve; imass 1 and 3, and one case periodic in t
+    (ibound = +1)."""
     from chiralheisenberg_amd import checkpoint
     g = np.load(os.path.join(HERE, "golden", "ref_fermion_l12_ls8.npz"))
     chi, _, _ = checkpoint.read_con(os.path.join(HERE, "golden", "con_l12_fp64"), kvol=1728)
-    lat = orc.Lattice(12, 8)
-    for k, tol in ((0, 5e-9), (1, 1e-5)):
+    cases = list(fermion_cases(g))
+    assert len(cases) == 3 and cases[2][1] == 1
+    for k, ibound, tol in cases:
+        lat = orc.Lattice(12, 8, ibound=ibound)
         rng = orc.Rng(float(g["rng_seed"]))
         rng.rano(0)
         ittt0 = int(12 * rng.rano(0)) + 1
