@@ -63,6 +63,27 @@ def heatbath_source(lat, rng):
     return R
 
 
+def leapfrog(lat, Phi, chit, pp, par, remez2g, stop):
+    """The molecular-dynamics evolution of rhmc/cheisdwf.f:261-315, in place
+    on chit and pp: a half momentum step, then steps of chit += dt pp and the
+    force; after each step's force stop(step) decides the end, and the last
+    momentum step is a half step (a full one when itermax ends the loop).
+    The reference's stop is one rano(yrun) < 1/iterl draw per step.  Returns
+    the number of steps."""
+    dt = par["dt"]
+    args = (par["rescgg"], chit, par["beta"], remez2g, par["am"], par["hpin"], par["imass"])
+    F, _, _ = fm.force(lat, Phi, *args)
+    pp -= 0.5 * dt * F
+    for it in range(1, par["itermax"] + 1):
+        chit += dt * pp
+        F, _, _ = fm.force(lat, Phi, *args)
+        if stop(it):
+            pp -= 0.5 * dt * F
+            return it
+        pp -= dt * F
+    return par["itermax"]
+
+
 def trajectory(lat, chi, rng, isweep, par=None, sets=None, measure_draw=True):
     """One RHMC trajectory; chi (float64 (3, kvol)) is replaced in place when
     the proposal is accepted.  Returns a dict of H0, H1, dH, accepted, the
@@ -85,19 +106,8 @@ def trajectory(lat, chi, rng, isweep, par=None, sets=None, measure_draw=True):
     pp[2] = ps[0]
     (H0, hg0, hp0, S0), _, it_h0 = fm.hamilton(lat, Phi, p["rescga"], chit, pp, p["beta"], sets["remez2"],
                                                p["am"], p["hpin"], p["imass"], isweep)
-    F, _, _ = fm.force(lat, Phi, p["rescgg"], chit, p["beta"], sets["remez2g"], p["am"], p["hpin"], p["imass"])
-    pp -= 0.5 * p["dt"] * F
     proby = 1.0 / p["iterl"]
-    steps = 0
-    for it in range(1, p["itermax"] + 1):
-        chit += p["dt"] * pp
-        F, _, _ = fm.force(lat, Phi, p["rescgg"], chit, p["beta"], sets["remez2g"], p["am"], p["hpin"],
-                           p["imass"])
-        steps = it
-        if rng.rano(1) < proby:
-            pp -= 0.5 * p["dt"] * F
-            break
-        pp -= p["dt"] * F
+    steps = leapfrog(lat, Phi, chit, pp, p, sets["remez2g"], lambda it: rng.rano(1) < proby)
     (H1, hg1, hp1, S1), _, it_h1 = fm.hamilton(lat, Phi, p["rescga"], chit, pp, p["beta"], sets["remez2"],
                                                p["am"], p["hpin"], p["imass"], isweep)
     dH = H0 - H1

@@ -118,3 +118,58 @@ def test_driver_run_matches_oracle(tmp_path):
     assert seed_end == orng.get()
     assert out["acceptances"] == sum(a for _, a in control)
     assert len(open(tmp_path / "fort.14").readlines()) == iter2 // iprint
+
+
+def _md_start(L, Ls, seed):
+    """A lattice, chi ~ N(0, 1/beta), Gaussian momenta and a heatbath Phi."""
+    from conftest import spinor
+    rs = np.random.RandomState(seed)
+    lat = fm.Lattice(L, Ls)
+    chi0 = rs.randn(3, lat.kvol) / np.sqrt(0.475)
+    pp0 = rs.randn(3, lat.kvol)
+    a4, ad4, _, _ = remez.load("remez4")
+    Phi, _ = fm.qmrherm(lat, spinor(rs, lat.shape), 1e-12, a4, ad4, chi0, 0.0, 0.0, 1, 0)
+    return lat, chi0, pp0, Phi
+
+
+def test_leapfrog_reversible():
+    """The MD evolution (rhmc/cheisdwf.f:261-315) is time-reversible: n steps
+    forward, momenta reversed, n steps back return chi and the momenta to
+    their start, to what the force solves' tolerance (1e-13) allows."""
+    lat, chi0, pp0, Phi = _md_start(4, 4, 7)
+    par = dict(rhmc.DEFAULTS, dt=0.1, rescgg=1e-13)
+    r2g = remez.load("remez2g")
+    chit, pp = chi0.copy(), pp0.copy()
+    assert rhmc.leapfrog(lat, Phi, chit, pp, par, r2g, lambda it: it == 8) == 8
+    moved = np.abs(chit - chi0).max()
+    pp = -pp
+    assert rhmc.leapfrog(lat, Phi, chit, pp, par, r2g, lambda it: it == 8) == 8
+    back = np.abs(chit - chi0).max() / np.abs(chi0).max()
+    pback = np.abs(pp + pp0).max() / np.abs(pp0).max()
+    print("moved %.3f, return error chi %.2e, momenta %.2e" % (moved, back, pback))
+    assert moved > 0.1
+    assert back <= 1e-10 and pback <= 1e-10
+    lat.close()
+
+
+def test_leapfrog_dH_second_order():
+    """|dH| over a trajectory of fixed length 0.4 falls as dt^2 (leapfrog is
+    second order): halving dt divides it by ~4.  The force uses the remez2
+    set here, the Hamiltonian's own, so that no mismatch of two rational
+    approximations puts a floor under dH."""
+    lat, chi0, pp0, Phi = _md_start(4, 4, 8)
+    r2 = remez.load("remez2")
+
+    def dH(dt):
+        n = int(round(0.4 / dt))
+        par = dict(rhmc.DEFAULTS, dt=dt, rescgg=1e-12)
+        chit, pp = chi0.copy(), pp0.copy()
+        (h0, *_), _, _ = fm.hamilton(lat, Phi, 1e-12, chit, pp, 0.475, r2, 0.0, 0.0, 1, 1)
+        assert rhmc.leapfrog(lat, Phi, chit, pp, par, r2, lambda it: it == n) == n
+        (h1, *_), _, _ = fm.hamilton(lat, Phi, 1e-12, chit, pp, 0.475, r2, 0.0, 0.0, 1, 1)
+        return h1 - h0
+
+    d = [dH(dt) for dt in (0.1, 0.05, 0.025)]
+    print("dH at dt 0.1, 0.05, 0.025:", d)
+    assert 3.0 < d[0] / d[1] < 5.0 and 3.0 < d[1] / d[2] < 5.0, d
+    lat.close()
