@@ -971,6 +971,7 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
         double stop_fac = 0.5;
         h.stop2 = stop_fac * target2;
         h.maxiter = maxiter;
+        h.aslot = kAlphaEmpty;
         HIPCHK(c, hipMemsetAsync(xs, 0, n * ba, c->stream));
         if constexpr (S::scaled) {
             // p = rs / sigma, sigma = rms of the residual (p starts at zero so
@@ -990,25 +991,23 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
                                (const typename SA::st *)rs);
             HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
         }
-        // the fused CG stencil multiplies the stored M^dagger M p by alpha
-        // (alpha sigma for scaled storage)
-        const double *alpha_ptr = S::scaled ? &c->cg->alpha_s : &c->cg->alpha;
         const int *skip = &c->cg->halt;
         // an even number of iterations per chunk: a chunk starts at an even
         // iteration, so its p buffers are the same in every replay
         const int chunk = std::max(2, chunk_for(c, 10.0 * bs * n) & ~1);
         typedef typename S::st sst;
         typedef typename SA::st ast;
-        // iteration k (parity par = k % 2): five launches
+        // iteration k (parity par = k % 2): four launches.  The fused CG
+        // stencil forms alpha itself (its workgroup 0, from the M p launch's
+        // partials; the stored M^dagger M p is multiplied by alpha, or alpha
+        // sigma for scaled storage) and applies r -= alpha M^dagger M p; the
+        // x increments are in the p update
         auto enqueue_iter = [&](int par) {
             void *pc = pbuf[par], *po = pbuf[par ^ 1];
             dslash_launch<S, SA, false, EPI_NORM>(c, mp, pc, chi, hpin, imass,
                                                  epi(c->partial, nullptr, nullptr, nullptr, nullptr, skip));
-            hipLaunchKernelGGL(k_rcg_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->cg, (const double *)c->partial,
-                               np);
-            // r -= alpha M^dagger M p (the x increments are in the p update)
-            dslash_launch<S, SA, true, EPI_CG>(c, nullptr, mp, chi, hpin, imass,
-                                              epi(c->partial, alpha_ptr, pc, xs, rs, skip, 1));
+            dslash_launch<S, SA, true, EPI_CGW>(c, nullptr, mp, chi, hpin, imass,
+                                               epi(c->partial, nullptr, c->cg, nullptr, rs, skip, 1));
             hipLaunchKernelGGL(k_rcg_beta, dim3(1), dim3(kBlock), 0, c->stream, c->cg, (const double *)c->partial,
                                np);
             if (par == 0)
@@ -1051,6 +1050,11 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipMemcpyAsync(&h, c->cg, sizeof h, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (h.aerr) {
+                status = fail(c, CHS_EHIP, "CG stencil: alpha not published within the wait bound (iteration %d)",
+                              h.iter);
+                break;
+            }
             if (!h.hold) continue;
             // iteration k raised the hold; its p update did not run.  Add the
             // pending increments (a_k p_k, and for odd k a_{k-1} p_{k-1}), then
@@ -1126,6 +1130,7 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
             h.maxr2 = rn2;
             h.hold = 0;
             h.halt = 0;
+            h.aslot = kAlphaEmpty;
             HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
             hipLaunchKernelGGL((k_cg_prestart<S, SA>), dim3(gn), dim3(kBlock), 0, c->stream, n,
                                (const sst *)pbuf[par], (sst *)pbuf[par ^ 1], (const ast *)rs, (const CgState *)c->cg);

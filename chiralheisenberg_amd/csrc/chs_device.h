@@ -171,7 +171,9 @@ enum Epi {
     EPI_DOT = 3,     // out = M in, partial += Re <aux0, out>
     EPI_CG = 4,      // no store: aux1 (x) += alpha aux0 (p), aux2 (r) -= alpha (M in),
                      // partial += |r|^2; alpha = scal[0]; flag bit0: skip x update
-    EPI_CGQ = 5      // EPI_CG that also stores out = M in (flexible CG keeps A p)
+    EPI_CGQ = 5,     // EPI_CG that also stores out = M in (flexible CG keeps A p)
+    EPI_CGW = 6      // EPI_CG without the x update, alpha formed inside the launch:
+                     // aux0 = CgState (cgw_publish / cgw_wait), aux2 = r
 };
 
 struct EpiArgs {
@@ -641,7 +643,59 @@ struct CgState {
     int iter;            // iterations taken
     int maxiter;
     int scaled;          // p and M p are stored scaled (H16)
+    // the fused CG stencil's multiplier (alpha, or alpha sigma) as published
+    // by its workgroup 0, kAlphaEmpty until then (EPI_CGW)
+    unsigned long long aslot;
+    int aerr;            // a wave gave up waiting for aslot
+    int pad_;
 };
+
+// In-launch alpha of the reliable-update CG (EPI_CGW).  Workgroup 0 of the
+// fused CG stencil sums the |M p|^2 partials the previous launch left, forms
+// alpha = rnorm2 / |M p|^2 (scaled storage: alpha sigma as the multiplier)
+// and publishes the multiplier in st->aslot with one agent-scope atomic
+// store; every other wave, once its stencil work is done, reads st->aslot
+// with agent-scope atomic loads until it is published.  Workgroup 0 is
+// dispatched first and waits for nothing, so the wait ends; it is still
+// bounded (kAlphaSpin polls of ~1 us, about a second), after which the wave
+// sets st->aerr and uses NaN, so that every wave exits.  k_rcg_beta puts
+// kAlphaEmpty back for the next iteration.  The partials are read before any
+// workgroup of the launch can overwrite them: each writes its own partial
+// only after it has read the published alpha.
+constexpr unsigned long long kAlphaEmpty = 0x7FF4DEADBEEF0001ull;   // a signalling NaN arithmetic never makes
+constexpr int kAlphaSpin = 1 << 20;
+
+__device__ __forceinline__ void cgw_publish(CgState *st, const double *__restrict__ partial, int np) {
+    const int scaled = st->scaled;
+    const double rn = st->rnorm2, sig = st->sigma;
+    const double t = sum_partials(partial, np);
+    if (threadIdx.x != 0) return;
+    double m;
+    if (scaled) {
+        // t = |M p|^2 / sigma^2
+        const double a = rn / (sig * sig * t);
+        st->alpha = a;
+        st->alpha_s = a * sig;
+        m = a * sig;
+    } else {
+        m = rn / t;
+        st->alpha = m;
+    }
+    __hip_atomic_store(&st->aslot, __builtin_bit_cast(unsigned long long, m), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double cgw_wait(CgState *st) {
+    for (int n = 0;; n++) {
+        const unsigned long long v = __hip_atomic_load(&st->aslot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (v != kAlphaEmpty) return __builtin_bit_cast(double, v);
+        if (n >= kAlphaSpin) {
+            __hip_atomic_store(&st->aerr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return __builtin_nan("");
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
 
 // reference CGNR: alpha = alphan / |M p|^2
 __global__ void k_cg_alpha(CgState *st, const double *__restrict__ partial, int np) {
@@ -800,32 +854,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_xflush(long n, const typename S::
     }
 }
 
-// Reliable-update CG scalars.  alpha = rnorm2 / |M p|^2; after |r_s|^2 of
-// the iterated residual either a plain step (beta) or a hold for the host
-// (reliable update due, converged candidate, iteration cap, or a non-finite
-// value).  The kernels read the state together with the partials (one
-// memory round trip instead of three: halt flag, partials, then the fields)
-// and test halt only before storing.
-__global__ void k_rcg_alpha(CgState *st, const double *__restrict__ partial, int np) {
-    const int halt = st->halt, scaled = st->scaled;
-    const double rn = st->rnorm2, sig = st->sigma;
-    const double t = sum_partials(partial, np);
-    if (threadIdx.x != 0 || halt) return;
-    if (scaled) {
-        // t = |M p|^2 / sigma^2
-        const double a = rn / (sig * sig * t);
-        st->alpha = a;
-        st->alpha_s = a * sig;
-    } else {
-        st->alpha = rn / t;
-    }
-}
-
+// Reliable-update CG scalars after |r_s|^2 of the iterated residual (alpha =
+// rnorm2 / |M p|^2 is formed inside the fused CG stencil, cgw_publish):
+// either a plain step (beta) or a hold for the host (reliable update due,
+// converged candidate, iteration cap, or a non-finite value).  The kernel
+// reads the state together with the partials (one memory round trip instead
+// of three: halt flag, partials, then the fields) and tests halt only before
+// storing.
 __global__ void k_rcg_beta(CgState *st, const double *__restrict__ partial, int np) {
     const int halt = st->halt, iter = st->iter + 1, maxiter = st->maxiter, scaled = st->scaled;
     const double rn_old = st->rnorm2, d2 = st->delta2, maxr2 = st->maxr2, stop2 = st->stop2;
     const double sig = st->sigma, alpha_s = st->alpha_s, nelem = st->nelem, xpa = st->xp_a, alpha = st->alpha;
     const double rn = sum_partials(partial, np);
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&st->aslot, kAlphaEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x != 0 || halt) return;
     st->iter = iter;
     // hold for the host: reliable update due, converged candidate, iteration

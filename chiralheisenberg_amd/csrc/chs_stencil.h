@@ -275,6 +275,8 @@ __device__ __forceinline__ void dsl_block(const Lat &lat, typename S::st *__rest
         }
         cx<T> o[1][4], c[1][4];
         stencil<S, DAG, 1, LAYOUT>(lat, in, chi, hpin, imass, s, site, fb, o, c);
+        TA alpha_w = TA(0);
+        if constexpr (EPI == EPI_CGW) alpha_w = (TA)cgw_wait((CgState *)ea.aux0);
         double acc = 0.0;
 #pragma unroll
         for (int d = 0; d < 4; d++) {
@@ -283,12 +285,12 @@ __device__ __forceinline__ void dsl_block(const Lat &lat, typename S::st *__rest
                 const T ib = (T)(1.0 / ea.scal[0]);
                 store_nt<S>(out + po, g, cscale(ib, o[0][d]));
                 store_nt<S>((typename S::st *)ea.aux0 + po, g, cscale(ib, c[0][d]));
-            } else if (EPI == EPI_CG || EPI == EPI_CGQ) {
+            } else if (EPI == EPI_CG || EPI == EPI_CGQ || EPI == EPI_CGW) {
                 if (EPI == EPI_CGQ) store_nt<S>(out + po, g, o[0][d]);
-                const TA alpha = (TA)ea.scal[0];
+                const TA alpha = EPI == EPI_CGW ? alpha_w : (TA)ea.scal[0];
                 typename SA::st *xr = (typename SA::st *)ea.aux1 + po;
                 typename SA::st *rr = (typename SA::st *)ea.aux2 + po;
-                if (!(ea.flag & 1)) {
+                if (EPI != EPI_CGW && !(ea.flag & 1)) {
                     const cx<T> pv = S::ld((const typename S::st *)ea.aux0 + po, g);
                     cx<TA> xv = SA::ld(xr, g);
                     xv = {xv.r + alpha * (TA)pv.r, xv.i + alpha * (TA)pv.i};
@@ -306,7 +308,7 @@ __device__ __forceinline__ void dsl_block(const Lat &lat, typename S::st *__rest
         }
         part = acc;
     }
-    if (EPI == EPI_NORM || EPI == EPI_DOT || EPI == EPI_CG || EPI == EPI_CGQ) {
+    if (EPI == EPI_NORM || EPI == EPI_DOT || EPI == EPI_CG || EPI == EPI_CGQ || EPI == EPI_CGW) {
         const double t = block_sum(part);
         if (threadIdx.x == 0) ea.partial[lb] = t;
     }
@@ -317,6 +319,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     Lat lat, typename S::st *__restrict__ out, const typename S::st *__restrict__ in, const double *__restrict__ chi,
     double hpin, int imass, EpiArgs ea) {
     if (ea.skip && *ea.skip) return;
+    if constexpr (EPI == EPI_CGW)
+        if (blockIdx.x == 0) cgw_publish((CgState *)ea.aux0, ea.partial, (int)gridDim.x);
     dsl_block<S, SA, DAG, EPI, LAYOUT>(lat, out, in, chi, hpin, imass, ea, logical_block_x());
 }
 
@@ -467,6 +471,8 @@ __device__ __forceinline__ void dsl_block_pair(const Lat &lat, typename S::st *_
         }
         cx<T> o0[4], o1[4], c0[4], c1[4];
         stencil_pair<S, DAG>(lat, in, chi, hpin, imass, s, site, fb, o0, o1, c0, c1);
+        TA alpha_w = TA(0);
+        if constexpr (EPI == EPI_CGW) alpha_w = (TA)cgw_wait((CgState *)ea.aux0);
         double acc = 0.0;
 #pragma unroll
         for (int d = 0; d < 4; d++) {
@@ -475,12 +481,12 @@ __device__ __forceinline__ void dsl_block_pair(const Lat &lat, typename S::st *_
                 const T ib = (T)(1.0 / ea.scal[0]);
                 store_nt2<S>(out + po, g, S::pack2(cscale(ib, o0[d]), cscale(ib, o1[d])));
                 store_nt2<S>((typename S::st *)ea.aux0 + po, g, S::pack2(cscale(ib, c0[d]), cscale(ib, c1[d])));
-            } else if (EPI == EPI_CG || EPI == EPI_CGQ) {
+            } else if (EPI == EPI_CG || EPI == EPI_CGQ || EPI == EPI_CGW) {
                 if (EPI == EPI_CGQ) store_nt2<S>(out + po, g, S::pack2(o0[d], o1[d]));
-                const TA alpha = (TA)ea.scal[0];
+                const TA alpha = EPI == EPI_CGW ? alpha_w : (TA)ea.scal[0];
                 typename SA::st *xr = (typename SA::st *)ea.aux1 + po;
                 typename SA::st *rr = (typename SA::st *)ea.aux2 + po;
-                if (!(ea.flag & 1)) {
+                if (EPI != EPI_CGW && !(ea.flag & 1)) {
                     cx<T> pa, pb;
                     S::ld2((const typename S::st *)ea.aux0 + po, g, pa, pb);
                     cx<TA> xa, xb;
@@ -505,7 +511,7 @@ __device__ __forceinline__ void dsl_block_pair(const Lat &lat, typename S::st *_
         }
         part = acc;
     }
-    if (EPI == EPI_NORM || EPI == EPI_DOT || EPI == EPI_CG || EPI == EPI_CGQ) {
+    if (EPI == EPI_NORM || EPI == EPI_DOT || EPI == EPI_CG || EPI == EPI_CGQ || EPI == EPI_CGW) {
         const double t = block_sum(part);
         if (threadIdx.x == 0) ea.partial[lb] = t;
     }
@@ -517,6 +523,8 @@ __global__ __launch_bounds__(kBlock) void k_dsl_pair(Lat lat, typename S::st *__
                                                      const double *__restrict__ chi, double hpin, int imass,
                                                      EpiArgs ea) {
     if (ea.skip && *ea.skip) return;
+    if constexpr (EPI == EPI_CGW)
+        if (blockIdx.x == 0) cgw_publish((CgState *)ea.aux0, ea.partial, (int)gridDim.x);
     dsl_block_pair<S, SA, DAG, EPI>(lat, out, in, chi, hpin, imass, ea, logical_block_x());
 }
 
