@@ -394,8 +394,9 @@ def main():
                                   "traffic": measured_traffic("cg_stencil_" + cg_form, args.L, args.Ls),
                                   "kernel": cg_kernel, "us": ms_cg * 1e3,
                                   "bytes_per_launch": cg_bytes,
-                                  "note": "isolated, 4 rotating buffer sets (cache-cold); inside the solve its "
-                                          "working set stays in the 256 MB Infinity Cache"},
+                                  "note": "isolated, 4 rotating buffer sets (cache-cold), alpha read from memory; "
+                                          "the solve runs the same stencil with alpha formed inside the launch "
+                                          "(EPI_CGW), and its working set stays in the 256 MB Infinity Cache"},
     }
     if not args.no_extras:
         out["learned"] = learned_bench(fm, lat, chi_d, b_d, args.tol)
