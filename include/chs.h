@@ -216,7 +216,9 @@ typedef struct {
  * Returns CHS_ENOCONV after maxiter iterations (maxiter <= 0: 100000), on a
  * non-finite residual, or when the FP64 true residual has not improved in 8
  * reliable updates in a row (a diverging low-precision solve: BF16 storage
- * at L=16, Ls=32, beta=0.45 is one). */
+ * at L=16, Ls=32, beta=0.45 is one).  The CG step length alpha is formed
+ * inside the fused CG stencil launch, whose waves wait (bounded, about a
+ * second) for it: CHS_EHIP if a wave gave up. */
 int chs_solve_mdagm_dev(chs_ctx *ctx, const double *b, double *x,
                         const double *chi, double hpin, int imass, double tol,
                         int prec, double delta, int maxiter,
