@@ -189,7 +189,7 @@ void dslash_launch(chs_ctx *c, void *out, const void *in, const double *chi, dou
 }
 
 EpiArgs epi(double *partial = nullptr, const double *scal = nullptr, void *a0 = nullptr, void *a1 = nullptr,
-            void *a2 = nullptr, const int *skip = nullptr, int flag = 0) {
+            void *a2 = nullptr, const int *skip = nullptr, int flag = 0, CgState *cgs = nullptr) {
     EpiArgs e;
     e.partial = partial;
     e.scal = scal;
@@ -198,6 +198,7 @@ EpiArgs epi(double *partial = nullptr, const double *scal = nullptr, void *a0 = 
     e.aux2 = a2;
     e.skip = skip;
     e.flag = flag;
+    e.cgs = cgs;
     return e;
 }
 
@@ -608,6 +609,7 @@ int chs_congrad_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
     h.betad = 1.0;
     h.resid = (double)n * res * res;
     h.maxiter = (int)(c->Ls * c->kvol);
+    h.aslot = kAlphaEmpty;
     HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(p, x, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(r, phi, 16 * n, hipMemcpyDeviceToDevice, c->stream));
@@ -624,12 +626,12 @@ int chs_congrad_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
                 dslash_launch<F64, F64, true, EPI_CG>(c, nullptr, x1, chi, hpin, imass,
                                                      epi(c->partial, &c->cg->alpha, p, x, r, skip, 1));
             } else {
+                // alpha = alphan / |M p|^2 (alphan = |r|^2 = rnorm2) inside the
+                // fused stencil, from the M p launch's partials
                 dslash_launch<F64, F64, false, EPI_NORM>(c, x1, p, chi, hpin, imass,
                                                         epi(c->partial, nullptr, nullptr, nullptr, nullptr, skip));
-                hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kBlock), 0, c->stream, c->cg,
-                                   (const double *)c->partial, np);
-                dslash_launch<F64, F64, true, EPI_CG>(c, nullptr, x1, chi, hpin, imass,
-                                                     epi(c->partial, &c->cg->alpha, p, x, r, skip, 0));
+                dslash_launch<F64, F64, true, EPI_CGW>(c, nullptr, x1, chi, hpin, imass,
+                                                      epi(c->partial, nullptr, p, x, r, skip, 0, c->cg));
             }
             hipLaunchKernelGGL(k_cg_beta_ref, dim3(1), dim3(kBlock), 0, c->stream, c->cg, (const double *)c->partial,
                                np, nx);
@@ -643,6 +645,8 @@ int chs_congrad_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
     HIPCHK(c, hipMemcpyAsync(&fin, c->cg, sizeof fin, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (itercg) *itercg = fin.iter;
+    if (fin.aerr) return fail(c, CHS_EHIP, "congrad: alpha not published within the wait bound (iteration %d)",
+                              fin.iter);
     if (!(fin.rnorm2 < fin.resid))
         return fail(c, CHS_ENOCONV, "# iterations of congrad exceeds niterc (%d)", fin.iter);
     return CHS_OK;
@@ -1007,7 +1011,7 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
             dslash_launch<S, SA, false, EPI_NORM>(c, mp, pc, chi, hpin, imass,
                                                  epi(c->partial, nullptr, nullptr, nullptr, nullptr, skip));
             dslash_launch<S, SA, true, EPI_CGW>(c, nullptr, mp, chi, hpin, imass,
-                                               epi(c->partial, nullptr, c->cg, nullptr, rs, skip, 1));
+                                               epi(c->partial, nullptr, pc, xs, rs, skip, 1, c->cg));
             hipLaunchKernelGGL(k_rcg_beta, dim3(1), dim3(kBlock), 0, c->stream, c->cg, (const double *)c->partial,
                                np);
             if (par == 0)

@@ -172,8 +172,8 @@ enum Epi {
     EPI_CG = 4,      // no store: aux1 (x) += alpha aux0 (p), aux2 (r) -= alpha (M in),
                      // partial += |r|^2; alpha = scal[0]; flag bit0: skip x update
     EPI_CGQ = 5,     // EPI_CG that also stores out = M in (flexible CG keeps A p)
-    EPI_CGW = 6      // EPI_CG without the x update, alpha formed inside the launch:
-                     // aux0 = CgState (cgw_publish / cgw_wait), aux2 = r
+    EPI_CGW = 6      // EPI_CG with alpha formed inside the launch from the previous
+                     // launch's |M p|^2 partials (cgs = CgState: cgw_publish / cgw_wait)
 };
 
 struct EpiArgs {
@@ -182,6 +182,7 @@ struct EpiArgs {
     void *aux0, *aux1, *aux2;
     const int *skip;   // if not NULL and *skip != 0 the launch does nothing
     int flag;
+    void *cgs;         // EPI_CGW: the CgState that receives and publishes alpha
 };
 
 // ---- deterministic FP64 reductions -----------------------------------------
@@ -697,13 +698,6 @@ __device__ __forceinline__ double cgw_wait(CgState *st) {
     }
 }
 
-// reference CGNR: alpha = alphan / |M p|^2
-__global__ void k_cg_alpha(CgState *st, const double *__restrict__ partial, int np) {
-    if (st->halt) return;
-    const double t = sum_partials(partial, np);
-    if (threadIdx.x == 0) st->alpha = st->alphan / t;
-}
-
 // reference CGNR after |r|^2 (rhmc/cheisdwf.f:1130-1148).  The reference
 // updates p before its convergence test; p is not used after convergence, so
 // halting before the p update leaves x and the iteration count unchanged.
@@ -711,6 +705,7 @@ __global__ void k_cg_beta_ref(CgState *st, const double *__restrict__ partial, i
     if (st->halt) return;
     const double betan = sum_partials(partial, np);
     if (threadIdx.x == 0) {
+        __hip_atomic_store(&st->aslot, kAlphaEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         double beta = betan / st->betad;
         st->betad = betan;
         st->alphan = betan;
@@ -718,7 +713,9 @@ __global__ void k_cg_beta_ref(CgState *st, const double *__restrict__ partial, i
         st->beta = beta;
         st->iter = nx;
         st->rnorm2 = betan;
-        if (betan < st->resid || nx >= st->maxiter) {
+        // a wave of the fused stencil that gave up waiting for alpha (aerr)
+        // also ends the solve; the host reports it
+        if (betan < st->resid || nx >= st->maxiter || st->aerr) {
             st->done = 1;
             st->halt = 1;
         }

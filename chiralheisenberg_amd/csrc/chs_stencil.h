@@ -276,7 +276,7 @@ __device__ __forceinline__ void dsl_block(const Lat &lat, typename S::st *__rest
         cx<T> o[1][4], c[1][4];
         stencil<S, DAG, 1, LAYOUT>(lat, in, chi, hpin, imass, s, site, fb, o, c);
         TA alpha_w = TA(0);
-        if constexpr (EPI == EPI_CGW) alpha_w = (TA)cgw_wait((CgState *)ea.aux0);
+        if constexpr (EPI == EPI_CGW) alpha_w = (TA)cgw_wait((CgState *)ea.cgs);
         double acc = 0.0;
 #pragma unroll
         for (int d = 0; d < 4; d++) {
@@ -290,7 +290,7 @@ __device__ __forceinline__ void dsl_block(const Lat &lat, typename S::st *__rest
                 const TA alpha = EPI == EPI_CGW ? alpha_w : (TA)ea.scal[0];
                 typename SA::st *xr = (typename SA::st *)ea.aux1 + po;
                 typename SA::st *rr = (typename SA::st *)ea.aux2 + po;
-                if (EPI != EPI_CGW && !(ea.flag & 1)) {
+                if (!(ea.flag & 1)) {
                     const cx<T> pv = S::ld((const typename S::st *)ea.aux0 + po, g);
                     cx<TA> xv = SA::ld(xr, g);
                     xv = {xv.r + alpha * (TA)pv.r, xv.i + alpha * (TA)pv.i};
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     double hpin, int imass, EpiArgs ea) {
     if (ea.skip && *ea.skip) return;
     if constexpr (EPI == EPI_CGW)
-        if (blockIdx.x == 0) cgw_publish((CgState *)ea.aux0, ea.partial, (int)gridDim.x);
+        if (blockIdx.x == 0) cgw_publish((CgState *)ea.cgs, ea.partial, (int)gridDim.x);
     dsl_block<S, SA, DAG, EPI, LAYOUT>(lat, out, in, chi, hpin, imass, ea, logical_block_x());
 }
 
@@ -472,7 +472,7 @@ __device__ __forceinline__ void dsl_block_pair(const Lat &lat, typename S::st *_
         cx<T> o0[4], o1[4], c0[4], c1[4];
         stencil_pair<S, DAG>(lat, in, chi, hpin, imass, s, site, fb, o0, o1, c0, c1);
         TA alpha_w = TA(0);
-        if constexpr (EPI == EPI_CGW) alpha_w = (TA)cgw_wait((CgState *)ea.aux0);
+        if constexpr (EPI == EPI_CGW) alpha_w = (TA)cgw_wait((CgState *)ea.cgs);
         double acc = 0.0;
 #pragma unroll
         for (int d = 0; d < 4; d++) {
@@ -486,7 +486,7 @@ __device__ __forceinline__ void dsl_block_pair(const Lat &lat, typename S::st *_
                 const TA alpha = EPI == EPI_CGW ? alpha_w : (TA)ea.scal[0];
                 typename SA::st *xr = (typename SA::st *)ea.aux1 + po;
                 typename SA::st *rr = (typename SA::st *)ea.aux2 + po;
-                if (EPI != EPI_CGW && !(ea.flag & 1)) {
+                if (!(ea.flag & 1)) {
                     cx<T> pa, pb;
                     S::ld2((const typename S::st *)ea.aux0 + po, g, pa, pb);
                     cx<TA> xa, xb;
@@ -524,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_dsl_pair(Lat lat, typename S::st *__
                                                      EpiArgs ea) {
     if (ea.skip && *ea.skip) return;
     if constexpr (EPI == EPI_CGW)
-        if (blockIdx.x == 0) cgw_publish((CgState *)ea.aux0, ea.partial, (int)gridDim.x);
+        if (blockIdx.x == 0) cgw_publish((CgState *)ea.cgs, ea.partial, (int)gridDim.x);
     dsl_block_pair<S, SA, DAG, EPI>(lat, out, in, chi, hpin, imass, ea, logical_block_x());
 }
 
