@@ -23,6 +23,27 @@ from .rhmc import RandomStream, aux_observables, heatbath_source
 DEFAULTS = dict(dt=0.06, beta=0.45, am=0.0, imass=1, iterl=16, itermax=1000, rescga=1e-9, rescgg=1e-6)
 
 
+def leapfrog(lat, Phi, x0, chit, pp, par, stop):
+    """The sibling's molecular-dynamics evolution (mh/cheisdwf_hmc.f:140-330),
+    in place on chit, pp and the chronological guess x0: a half momentum
+    step, then steps of chit += dt pp and the force; after each step's force
+    stop(step) decides the end, and the last momentum step is a half step (a
+    full one when itermax ends the loop).  The sibling's stop is one
+    rano(yrun) < 1/iterl draw per step.  Returns (steps, force iterations)."""
+    dt = par["dt"]
+    F, its = fm.hmc_force(lat, Phi, x0, par["rescgg"], chit, par["beta"], par["imass"])
+    pp -= 0.5 * dt * F
+    for step in range(1, par["itermax"] + 1):
+        chit += dt * pp
+        F, it = fm.hmc_force(lat, Phi, x0, par["rescgg"], chit, par["beta"], par["imass"])
+        its += it
+        if stop(step):
+            pp -= 0.5 * dt * F
+            return step, its
+        pp -= dt * F
+    return par["itermax"], its
+
+
 def trajectory(lat, chi, rng, isweep, par=None):
     """One HMC trajectory; chi (float64 (3, kvol)) is replaced in place when
     the proposal is accepted.  Returns a dict of H0, H1, dH, accepted, the
@@ -41,21 +62,8 @@ def trajectory(lat, chi, rng, isweep, par=None):
     ps = rng.gaussp(kv)
     pp[2] = ps[0]
     (H0, hg0, hp0, S0), it_h0 = fm.hmc_hamilton(lat, Phi, x0, p["rescga"], chit, pp, p["beta"], p["imass"])
-    its = 0
-    F, it = fm.hmc_force(lat, Phi, x0, p["rescgg"], chit, p["beta"], p["imass"])
-    its += it
-    pp -= 0.5 * p["dt"] * F
     proby = 1.0 / p["iterl"]
-    steps = 0
-    for step in range(1, p["itermax"] + 1):
-        chit += p["dt"] * pp
-        F, it = fm.hmc_force(lat, Phi, x0, p["rescgg"], chit, p["beta"], p["imass"])
-        its += it
-        steps = step
-        if rng.rano(1) < proby:
-            pp -= 0.5 * p["dt"] * F
-            break
-        pp -= p["dt"] * F
+    steps, its = leapfrog(lat, Phi, x0, chit, pp, p, lambda step: rng.rano(1) < proby)
     (H1, hg1, hp1, S1), it_h1 = fm.hmc_hamilton(lat, Phi, x0, p["rescga"], chit, pp, p["beta"], p["imass"])
     dH = H0 - H1
     accepted = True

@@ -131,3 +131,58 @@ def test_hmc_run_files(tmp_path):
     assert len(open(tmp_path / "fort.14").readlines()) == 3
     chi, seed, _ = checkpoint.read_con(tmp_path / "con", kvol=64)
     assert np.all(np.isfinite(chi)) and seed != 693764523745.0
+
+
+def _hmc_md_start(seed):
+    """L=4, Ls=8 (the congrad cap Ls*kvol = 512 lies above the ~290
+    iterations these solves take), chi ~ N(0, 1/beta), Gaussian momenta,
+    Phi = M^dagger R."""
+    rs = np.random.RandomState(seed)
+    lat = fm.Lattice(4, 8)
+    chi0 = rs.randn(3, lat.kvol) / np.sqrt(0.45)
+    pp0 = rs.randn(3, lat.kvol)
+    Phi = fm.dslashd(lat, spinor(rs, lat.shape), chi0, 0.0, 0.0, 1)
+    return lat, chi0, pp0, Phi
+
+
+def test_hmc_leapfrog_reversible():
+    """The sibling's MD evolution is time-reversible: 8 steps forward,
+    momenta reversed, 8 back (the chronological guess carried on) return chi
+    and the momenta to their start, to what the force solves' tolerance
+    (1e-13) allows (the oracle: 8e-15 and 6e-14)."""
+    lat, chi0, pp0, Phi = _hmc_md_start(9)
+    par = dict(hmc.DEFAULTS, dt=0.06, rescgg=1e-13)
+    x0 = Phi.copy()
+    chit, pp = chi0.copy(), pp0.copy()
+    assert hmc.leapfrog(lat, Phi, x0, chit, pp, par, lambda s: s == 8)[0] == 8
+    moved = np.abs(chit - chi0).max()
+    pp = -pp
+    assert hmc.leapfrog(lat, Phi, x0, chit, pp, par, lambda s: s == 8)[0] == 8
+    back = np.abs(chit - chi0).max() / np.abs(chi0).max()
+    pback = np.abs(pp + pp0).max() / np.abs(pp0).max()
+    print("moved %.3f, return error chi %.2e, momenta %.2e" % (moved, back, pback))
+    assert moved > 0.1
+    assert back <= 1e-11 and pback <= 1e-11
+    lat.close()
+
+
+def test_hmc_leapfrog_dH_second_order():
+    """|dH| over a trajectory of fixed length 0.4 falls as dt^2: halving dt
+    from 0.05 to 0.025 to 0.0125 divides it by ~4 (the oracle: 4.61, 4.14).
+    The sibling's action has no rational approximation, so nothing but the
+    solver tolerance puts a floor under dH."""
+    lat, chi0, pp0, Phi = _hmc_md_start(9)
+
+    def dH(dt):
+        n = int(round(0.4 / dt))
+        par = dict(hmc.DEFAULTS, dt=dt, rescgg=1e-12)
+        chit, pp, x0 = chi0.copy(), pp0.copy(), Phi.copy()
+        (h0, *_), _ = fm.hmc_hamilton(lat, Phi, x0.copy(), 1e-12, chit, pp, 0.45, 1)
+        assert hmc.leapfrog(lat, Phi, x0, chit, pp, par, lambda s: s == n)[0] == n
+        (h1, *_), _ = fm.hmc_hamilton(lat, Phi, x0.copy(), 1e-12, chit, pp, 0.45, 1)
+        return h1 - h0
+
+    d = [dH(dt) for dt in (0.05, 0.025, 0.0125)]
+    print("dH at dt 0.05, 0.025, 0.0125:", d)
+    assert 3.5 < d[0] / d[1] < 5.0 and 3.5 < d[1] / d[2] < 5.0, d
+    lat.close()
