@@ -71,6 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_ms_dsl(MsGroup gr, int parity) {
     ea.aux1 = ea.aux2 = nullptr;
     ea.skip = nullptr;
     ea.flag = 0;
+    ea.cgs = nullptr;
     const unsigned lb = b - gr.s_dsl[j];
     if (!DAG) {
         ea.partial = nullptr;
