@@ -46,6 +46,7 @@ struct chs_ctx {
     struct NNState *nn = nullptr; // learned preconditioner (chs_learn_host.inc)
     int use_graphs = 1;           // replay solver iteration chunks from hipGraphs (CHS_GRAPHS=0: off)
     int pair_stencil = 1;         // paired-site FP32 stencil on even L (CHS_PAIRS=0: off)
+    int plane_stencil = 1;        // LDS-plane plain FP64 dslash when L^2 <= 1024 (CHS_PLANES=0: off)
     int force_prec = CHS_FP64;    // storage of the force's multi-shift solves (chs_set_force_precision)
 };
 static void nn_destroy(chs_ctx *c);
@@ -176,6 +177,15 @@ unsigned dsl_grid(const chs_ctx *c) {
 template <class S, class SA, bool DAG, int EPI>
 void dslash_launch(chs_ctx *c, void *out, const void *in, const double *chi, double hpin, int imass,
                    EpiArgs ea) {
+    if constexpr (std::is_same<S, F64>::value && EPI == EPI_STORE) {
+        // the plain FP64 operator: xy-planes staged in LDS when a plane fits a workgroup
+        if (c->plane_stencil && plane_stencil_fits(c->lat) && !(ea.skip)) {
+            hipLaunchKernelGGL((k_dsl_plane<S, DAG>), dim3(plane_grid(c->lat)), dim3(plane_block(c->lat)),
+                               4 * sizeof(typename S::st) * (size_t)c->L * c->L, c->stream, c->lat,
+                               (typename S::st *)out, (const typename S::st *)in, chi, hpin, imass);
+            return;
+        }
+    }
     if constexpr (std::is_same<S, F32>::value) {
         if (use_pairs<S>(c)) {
             hipLaunchKernelGGL((k_dsl_pair<S, SA, DAG, EPI>), dim3(stencil_grid_pair(c->n5)), dim3(kBlock), 0,
@@ -286,6 +296,7 @@ chs_ctx *chs_ctx_create(int L, int Lt, int Ls, int ibound, double am3, int devic
     c->lat.ps = c->n5;
     if (const char *g = getenv("CHS_GRAPHS")) c->use_graphs = atoi(g) != 0;
     if (const char *g = getenv("CHS_PAIRS")) c->pair_stencil = atoi(g) != 0;
+    if (const char *g = getenv("CHS_PLANES")) c->plane_stencil = atoi(g) != 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->lz, sizeof(LzState)) != hipSuccess || hipMalloc(&c->cg, sizeof(CgState)) != hipSuccess ||
         hipMalloc(&c->dscal, 64 * sizeof(double)) != hipSuccess ||
