@@ -402,7 +402,8 @@ def main():
         "dslash_us": {"fp64": ms * 1e3, "fp32": ms32 * 1e3, "bf16": ms16 * 1e3, "fp16": msh * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": measured_traffic("dslash", args.L, args.Ls),
-                     "kernel": ("k_dsl_plane<F64,false>" if args.L * args.L <= 1024 else
+                     "kernel": ("k_dsl_plane<F64,false>"
+                                if args.L * args.L <= 1024 and os.environ.get("CHS_PLANES", "1") != "0" else
                                 "k_dsl<F64,F64,false,EPI_STORE>"), "us": ms * 1e3, "bytes_per_launch": 2 * V + C,
                      "inputs": "random values (all-zero buffers run ~15%% faster on this hardware), cycled over "
                                "%d buffer sets so that they come from HBM, not the 256 MB Infinity Cache" % nb64,
