@@ -178,6 +178,27 @@ def test_pair_stencil_equals_one_site(prec, L, Ls, Lt, monkeypatch):
             assert np.array_equal(outs[0], outs[1]), (prec, imass, dagger, np.abs(outs[0] - outs[1]).max())
 
 
+@pytest.mark.parametrize("L,Ls,Lt", [(2, 2, 2), (5, 3, 4), (8, 8, 8), (24, 4, 6)])
+def test_plane_stencil_matches_one_site(L, Ls, Lt, monkeypatch):
+    """The plain FP64 operator stages xy-planes in LDS (L^2 <= 1024); with
+    CHS_PLANES=0 it runs the one-site stencil.  Both add the same terms in the
+    same order: outputs agree to rounding (1e-15 relative), with the wall
+    terms (both imass, hpin at site 0) and the antiperiodic t boundary."""
+    monkeypatch.setenv("CHS_PLANES", "0")
+    one = fm.Lattice(L, Ls, Lt)
+    monkeypatch.setenv("CHS_PLANES", "1")
+    plane = fm.Lattice(L, Ls, Lt)
+    rs = np.random.RandomState(17 * L + Ls)
+    R, chi = spinor(rs, one.shape), phi_field(rs, one.kvol)
+    for imass in (1, 3):
+        for dag in (False, True):
+            f = fm.dslashd if dag else fm.dslash
+            a, b = f(one, R, chi, 0.0, 0.7, imass), f(plane, R, chi, 0.0, 0.7, imass)
+            assert np.abs(a - b).max() <= 1e-15 * np.abs(a).max(), (L, Ls, Lt, imass, dag)
+    one.close()
+    plane.close()
+
+
 def test_odd_L_low_precision_uses_one_site_stencil():
     lat, olat = lattices(5, 4, 6)
     rs = np.random.RandomState(8)
