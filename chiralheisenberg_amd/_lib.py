@@ -73,6 +73,11 @@ def _declare(L):
         "chs_hamilton": ([v, v, v, v, v, v, d, i, d, d, i, v, v, d, v, v, i, v, ip], i),
         "chs_derivs": ([v, v, v, d, i, v], i),
         "chs_set_force_precision": ([v, i], i),
+        "chs_set_lanczos_cap": ([v, i], i),
+        "chs_mirror_register": ([v, v], i),
+        "chs_mirror_touch": ([v, v], i),
+        "chs_mirror_release": ([v, v], i),
+        "chs_transfer_bytes": ([v, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)], i),
         "chs_fermion": ([v, v, d, d, i, i, i, v, v, ip], i),
         "chs_hmc_force": ([v, v, v, d, d, i, v, d, v, ip], i),
         "chs_hmc_hamilton": ([v, v, v, d, d, i, v, v, d, v, ip], i),

@@ -48,6 +48,13 @@ struct chs_ctx {
     int pair_stencil = 1;         // paired-site FP32 stencil on even L (CHS_PAIRS=0: off)
     int plane_stencil = 1;        // LDS-plane plain FP64 dslash when L^2 <= 1024 (CHS_PLANES=0: off)
     int force_prec = CHS_FP64;    // storage of the force's multi-shift solves (chs_set_force_precision)
+    int lz_cap = 5000;            // multi-shift iteration cap (the reference's niterc)
+    // Device mirrors of read-only host spinors, keyed by host address: the
+    // copy on the device is current while its generation equals the host
+    // array's (the caller bumps that with chs_mirror_touch after writing it).
+    struct Mirror { const void *host; void *dev; unsigned long long gen, dev_gen; };
+    std::vector<Mirror> mirrors;
+    unsigned long long h2d_spinor_bytes = 0, d2h_spinor_bytes = 0;
 };
 static void nn_destroy(chs_ctx *c);
 
@@ -125,6 +132,7 @@ int ensure_partials(chs_ctx *c) {
 int h2d_spinor(chs_ctx *c, void *dst, const void *host) {
     int rc = ensure(c, c->b[B_STAGE], 16 * (size_t)c->kferm);
     if (rc) return rc;
+    c->h2d_spinor_bytes += 16ull * (unsigned long long)c->kferm;
     HIPCHK(c, hipMemcpyAsync(c->b[B_STAGE].p, host, 16 * c->kferm, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL((k_relayout<F64>), dim3((unsigned)((c->kferm + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                        c->stream, c->lat, (double2 *)dst, (const double2 *)c->b[B_STAGE].p, 0);
@@ -135,10 +143,37 @@ int h2d_spinor(chs_ctx *c, void *dst, const void *host) {
 int d2h_spinor(chs_ctx *c, void *host, const void *src) {
     int rc = ensure(c, c->b[B_STAGE], 16 * (size_t)c->kferm);
     if (rc) return rc;
+    c->d2h_spinor_bytes += 16ull * (unsigned long long)c->kferm;
     hipLaunchKernelGGL((k_relayout<F64>), dim3((unsigned)((c->kferm + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                        c->stream, c->lat, (double2 *)c->b[B_STAGE].p, (const double2 *)src, 1);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(host, c->b[B_STAGE].p, 16 * c->kferm, hipMemcpyDeviceToHost, c->stream));
+    return CHS_OK;
+}
+
+chs_ctx::Mirror *find_mirror(chs_ctx *c, const void *host) {
+    for (auto &m : c->mirrors)
+        if (m.host == host) return &m;
+    return nullptr;
+}
+
+// The device copy (native layout) of a read-only host spinor argument: a
+// registered mirror, uploaded only when the host array's generation has moved
+// since its last upload, or else an upload into the work buffer `fallback`.
+int in_spinor(chs_ctx *c, const double *host, int fallback, const double **dev) {
+    if (chs_ctx::Mirror *m = find_mirror(c, host)) {
+        if (m->dev_gen != m->gen) {
+            int rc = h2d_spinor(c, m->dev, host);
+            if (rc) return rc;
+            m->dev_gen = m->gen;
+        }
+        *dev = (const double *)m->dev;
+        return CHS_OK;
+    }
+    int rc = ensure(c, c->b[fallback], 16 * (size_t)c->kferm);
+    if (rc) return rc;
+    if ((rc = h2d_spinor(c, c->b[fallback].p, host))) return rc;
+    *dev = (const double *)c->b[fallback].p;
     return CHS_OK;
 }
 
@@ -324,6 +359,8 @@ void chs_ctx_destroy(chs_ctx *c) {
     if (c->shifts_p.p) hipFree(c->shifts_p.p);
     if (c->shifts_x.p) hipFree(c->shifts_x.p);
     if (c->contrib.p) hipFree(c->contrib.p);
+    for (auto &m : c->mirrors)
+        if (m.dev) hipFree(m.dev);
     if (c->partial) hipFree(c->partial);
     if (c->partial2) hipFree(c->partial2);
     if (c->lz) hipFree(c->lz);
@@ -426,13 +463,14 @@ int chs_dslash_dev(chs_ctx *c, void *out, const void *in, const double *chi, dou
 
 static int dslash_host(chs_ctx *c, double *phi, const double *r, const double *chi, double hpin, int imass,
                        int dagger) {
-    int rc = ensure_spinors(c, {B_R, B_V});
+    int rc = ensure_spinors(c, {B_V});
     if (rc) return rc;
     rc = ensure(c, c->b[B_CHI], sizeof(double) * 3 * c->kvol);
     if (rc) return rc;
-    if ((rc = h2d_spinor(c, c->b[B_R].p, r))) return rc;
+    const double *dr = nullptr;
+    if ((rc = in_spinor(c, r, B_R, &dr))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->b[B_CHI].p, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
-    rc = chs_dslash_dev(c, c->b[B_V].p, c->b[B_R].p, buf<double>(c, B_CHI), hpin, imass, dagger, CHS_FP64);
+    rc = chs_dslash_dev(c, c->b[B_V].p, dr, buf<double>(c, B_CHI), hpin, imass, dagger, CHS_FP64);
     if (rc) return rc;
     if ((rc = d2h_spinor(c, phi, c->b[B_V].p))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -497,7 +535,7 @@ int qmrherm_impl(chs_ctx *c, const double *phi, double res, int *itercg, double 
     hipLaunchKernelGGL(k_lz_init, dim3(1), dim3(kBlock), 0, c->stream, c->lz, (const double *)c->dscal, 1,
                        std::sqrt((double)n * res * res));
 
-    const int niterc = 5000;
+    const int niterc = c->lz_cap;
     const int np1 = (int)dsl_grid<S>(c), np2 = (int)nblocks(n);
     const int chunk = chunk_for(c, (12.0 + 2.0 * ndiag) * (double)sizeof(st) * n);
     const int *skip = &c->lz->done;
@@ -554,7 +592,14 @@ int qmrherm_impl(chs_ctx *c, const double *phi, double res, int *itercg, double 
                            ndiag, (const double *)c->contrib.p, dsdchi);
     }
     HIPCHK(c, hipGetLastError());
-    // the reference returns silently after niterc iterations (rhmc/cheisdwf.f:639)
+    // The reference returns silently after niterc iterations
+    // (rhmc/cheisdwf.f:639) and never tests its residual for NaN; here both
+    // are reported, with x and the force contributions written as the
+    // reference would leave them.
+    if (done == 0)
+        return fail(c, CHS_ENOCONV, "multi-shift solve reached its cap of %d iterations without converging", niterc);
+    if (done == 2)
+        return fail(c, CHS_ENOCONV, "multi-shift solve: non-finite residual norm at iteration %d", it);
     return CHS_OK;
 }
 
@@ -578,6 +623,45 @@ int chs_qmrherm_prec_dev(chs_ctx *c, const double *phi, double res, int *itercg,
     return fail(c, CHS_EINVAL, "multi-shift precision must be FP64 or FP32");
 }
 
+int chs_set_lanczos_cap(chs_ctx *c, int niterc) {
+    if (niterc < 0) return fail(c, CHS_EINVAL, "multi-shift iteration cap %d < 0", niterc);
+    c->lz_cap = niterc ? niterc : 5000;
+    return CHS_OK;
+}
+
+int chs_mirror_register(chs_ctx *c, const void *host) {
+    if (!host) return fail(c, CHS_EINVAL, "mirror of a NULL host array");
+    if (find_mirror(c, host)) return CHS_OK;
+    chs_ctx::Mirror m{host, nullptr, 1, 0};
+    HIPCHK(c, hipMalloc(&m.dev, 16 * (size_t)c->kferm));
+    c->mirrors.push_back(m);
+    return CHS_OK;
+}
+
+int chs_mirror_touch(chs_ctx *c, const void *host) {
+    chs_ctx::Mirror *m = find_mirror(c, host);
+    if (!m) return fail(c, CHS_EINVAL, "no mirror registered for host array %p", host);
+    m->gen++;
+    return CHS_OK;
+}
+
+int chs_mirror_release(chs_ctx *c, const void *host) {
+    for (size_t i = 0; i < c->mirrors.size(); i++) {
+        if (c->mirrors[i].host != host) continue;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipFree(c->mirrors[i].dev));
+        c->mirrors.erase(c->mirrors.begin() + (long)i);
+        return CHS_OK;
+    }
+    return fail(c, CHS_EINVAL, "no mirror registered for host array %p", host);
+}
+
+int chs_transfer_bytes(const chs_ctx *c, unsigned long long *h2d, unsigned long long *d2h) {
+    if (h2d) *h2d = c->h2d_spinor_bytes;
+    if (d2h) *d2h = c->d2h_spinor_bytes;
+    return CHS_OK;
+}
+
 int chs_set_force_precision(chs_ctx *c, int prec) {
     if (prec != CHS_FP64 && prec != CHS_FP32) return fail(c, CHS_EINVAL, "force precision must be FP64 or FP32");
     c->force_prec = prec;
@@ -588,23 +672,25 @@ int chs_qmrherm(chs_ctx *c, const double *phi, double res, int *itercg, double a
                 const double *anum, const double *aden, int ndiag, int iflag, const double *chi, double *x,
                 double *dsdchi) {
     (void)am;
-    int rc = ensure_spinors(c, {B_PHI, B_X});
+    int rc = ensure_spinors(c, {B_X});
     if (rc) return rc;
     if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
     if ((rc = ensure(c, c->b[B_F], 24 * c->kvol))) return rc;
-    if ((rc = h2d_spinor(c, c->b[B_PHI].p, phi))) return rc;
+    if (iflag == 1 && !dsdchi) return fail(c, CHS_EINVAL, "iflag=1 needs dsdchi");
+    const double *dphi = nullptr;
+    if ((rc = in_spinor(c, phi, B_PHI, &dphi))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->b[B_CHI].p, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
-    if (iflag == 1) {
-        if (!dsdchi) return fail(c, CHS_EINVAL, "iflag=1 needs dsdchi");
-        HIPCHK(c, hipMemcpyAsync(c->b[B_F].p, dsdchi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
-    }
-    rc = chs_qmrherm_dev(c, buf<double>(c, B_PHI), res, itercg, hpin, imass, anum, aden, ndiag, iflag,
-                         buf<double>(c, B_CHI), buf<double>(c, B_X), buf<double>(c, B_F));
-    if (rc) return rc;
+    if (iflag == 1) HIPCHK(c, hipMemcpyAsync(c->b[B_F].p, dsdchi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
+    const int rs = chs_qmrherm_dev(c, dphi, res, itercg, hpin, imass, anum, aden, ndiag, iflag,
+                                   buf<double>(c, B_CHI), buf<double>(c, B_X), buf<double>(c, B_F));
+    if (rs != CHS_OK && rs != CHS_ENOCONV) return rs;
+    // results are written also when the solve stopped at its cap
+    const std::string msg = c->err;
     if (x && (rc = d2h_spinor(c, x, c->b[B_X].p))) return rc;
     if (iflag == 1) HIPCHK(c, hipMemcpyAsync(dsdchi, c->b[B_F].p, 24 * c->kvol, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    return CHS_OK;
+    c->err = msg;
+    return rs;
 }
 
 // ---- CGNR with the reference recurrences --------------------------------------
@@ -754,17 +840,19 @@ int chs_fermion(chs_ctx *c, const double *chi, double res, double hpin, int imas
 int chs_congrad(chs_ctx *c, const double *phi, double res, int *itercg, double am, double hpin, int imass,
                 const double *chi, double *x) {
     (void)am;
-    int rc = ensure_spinors(c, {B_PHI, B_X});
+    int rc = ensure_spinors(c, {B_X});
     if (rc) return rc;
     if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
-    if ((rc = h2d_spinor(c, c->b[B_PHI].p, phi))) return rc;
+    const double *dphi = nullptr;
+    if ((rc = in_spinor(c, phi, B_PHI, &dphi))) return rc;
     if ((rc = h2d_spinor(c, c->b[B_X].p, x))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->b[B_CHI].p, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
-    int rc2 = chs_congrad_dev(c, buf<double>(c, B_PHI), res, itercg, hpin, imass, buf<double>(c, B_CHI),
-                              buf<double>(c, B_X));
+    int rc2 = chs_congrad_dev(c, dphi, res, itercg, hpin, imass, buf<double>(c, B_CHI), buf<double>(c, B_X));
     if (rc2 != CHS_OK && rc2 != CHS_ENOCONV) return rc2;
+    const std::string msg = c->err;
     if ((rc = d2h_spinor(c, x, c->b[B_X].p))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (rc2) c->err = msg;
     return rc2;
 }
 
@@ -773,13 +861,14 @@ int chs_force(chs_ctx *c, const double *phi, double res1, double am, double hpin
               double beta, const double *anum, const double *aden, const double *bnum, const double *bden, int ndiag,
               double *dsdchi, double *x, int *its) {
     (void)am;
-    int rc = ensure_spinors(c, {B_PHI, B_X, B_T2});
+    int rc = ensure_spinors(c, {B_X, B_T2});
     if (rc) return rc;
     if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
     if ((rc = ensure(c, c->b[B_F], 24 * c->kvol))) return rc;
-    double *dphi = buf<double>(c, B_PHI), *dx = buf<double>(c, B_X), *dx2 = buf<double>(c, B_T2);
+    double *dx = buf<double>(c, B_X), *dx2 = buf<double>(c, B_T2);
     double *dchi = buf<double>(c, B_CHI), *dF = buf<double>(c, B_F);
-    if ((rc = h2d_spinor(c, dphi, phi))) return rc;
+    const double *dphi = nullptr;
+    if ((rc = in_spinor(c, phi, B_PHI, &dphi))) return rc;
     HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_force_init, dim3(nblocks(3 * c->kvol)), dim3(kBlock), 0, c->stream, 3 * c->kvol, beta,
                        (const double *)dchi, dF);
@@ -788,11 +877,20 @@ int chs_force(chs_ctx *c, const double *phi, double res1, double am, double hpin
     // (rhmc/cheisdwf.f:500-513)
     // (solver storage per the context's force precision policy)
     auto ms = c->force_prec == CHS_FP32 ? qmrherm_impl<F32> : qmrherm_impl<F64>;
-    rc = ms(c, dphi, res1, &i1, hpin, imass, bnum, bden, ndiag, 0, dchi, dx, nullptr);
-    if (rc) return rc;
+    // a solve that stops at its cap (or on a non-finite residual) does not
+    // stop the force: the reference carries on, and the status is returned
+    // with the results written
+    std::string msg;
+    int rs = ms(c, dphi, res1, &i1, hpin, imass, bnum, bden, ndiag, 0, dchi, dx, nullptr);
+    if (rs != CHS_OK && rs != CHS_ENOCONV) return rs;
+    if (rs) msg = c->err;
     HIPCHK(c, hipMemcpyAsync(dx2, dx, 16 * c->kferm, hipMemcpyDeviceToDevice, c->stream));
     rc = ms(c, dx2, res1, &i2, hpin, imass, anum, aden, ndiag, 1, dchi, dx, dF);
-    if (rc) return rc;
+    if (rc != CHS_OK && rc != CHS_ENOCONV) return rc;
+    if (rc && !rs) {
+        rs = rc;
+        msg = c->err;
+    }
     HIPCHK(c, hipMemcpyAsync(dsdchi, dF, 24 * c->kvol, hipMemcpyDeviceToHost, c->stream));
     if (x && (rc = d2h_spinor(c, x, dx))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -800,19 +898,19 @@ int chs_force(chs_ctx *c, const double *phi, double res1, double am, double hpin
         its[0] = i1;
         its[1] = i2;
     }
-    return CHS_OK;
+    if (rs) c->err = msg;
+    return rs;
 }
 
 int chs_derivs(chs_ctx *c, const double *r, const double *x2, double anum, int imass, double *dsdchi) {
-    int rc = ensure_spinors(c, {B_T1, B_T2});
+    int rc = ensure(c, c->b[B_F], 24 * c->kvol);
     if (rc) return rc;
-    if ((rc = ensure(c, c->b[B_F], 24 * c->kvol))) return rc;
-    if ((rc = h2d_spinor(c, c->b[B_T1].p, r))) return rc;
-    if ((rc = h2d_spinor(c, c->b[B_T2].p, x2))) return rc;
+    const double *dr = nullptr, *dx2 = nullptr;
+    if ((rc = in_spinor(c, r, B_T1, &dr))) return rc;
+    if ((rc = in_spinor(c, x2, B_T2, &dx2))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->b[B_F].p, dsdchi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_derivs, dim3(nblocks(c->kvol)), dim3(kBlock), 0, c->stream, c->lat,
-                       (const double2 *)c->b[B_T1].p, (const double2 *)c->b[B_T2].p, anum, imass,
-                       buf<double>(c, B_F));
+    hipLaunchKernelGGL(k_derivs, dim3(nblocks(c->kvol)), dim3(kBlock), 0, c->stream, c->lat, (const double2 *)dr,
+                       (const double2 *)dx2, anum, imass, buf<double>(c, B_F));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(dsdchi, c->b[B_F].p, 24 * c->kvol, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -826,14 +924,15 @@ int chs_derivs(chs_ctx *c, const double *r, const double *x2, double anum, int i
 int chs_hmc_force(chs_ctx *c, const double *phi, double *x0, double res, double am, int imass, const double *chi,
                   double beta, double *dsdchi, int *itercg) {
     (void)am;
-    int rc = ensure_spinors(c, {B_PHI, B_X, B_Q0, B_T2});
+    int rc = ensure_spinors(c, {B_X, B_Q0, B_T2});
     if (rc) return rc;
     if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
     if ((rc = ensure(c, c->b[B_F], 24 * c->kvol))) return rc;
     const long n = c->kferm;
-    double *dphi = buf<double>(c, B_PHI), *dx1 = buf<double>(c, B_X), *dx0 = buf<double>(c, B_Q0);
+    double *dx1 = buf<double>(c, B_X), *dx0 = buf<double>(c, B_Q0);
     double *dx2 = buf<double>(c, B_T2), *dchi = buf<double>(c, B_CHI), *dF = buf<double>(c, B_F);
-    if ((rc = h2d_spinor(c, dphi, phi))) return rc;
+    const double *dphi = nullptr;
+    if ((rc = in_spinor(c, phi, B_PHI, &dphi))) return rc;
     if ((rc = h2d_spinor(c, dx0, x0))) return rc;
     HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(dx1, dx0, 16 * n, hipMemcpyDeviceToDevice, c->stream));
@@ -862,13 +961,14 @@ int chs_hmc_force(chs_ctx *c, const double *phi, double *x0, double res, double 
 int chs_hmc_hamilton(chs_ctx *c, const double *phi, double *x0, double res, double am, int imass, const double *chi,
                      const double *pp, double beta, double *h4, int *itercg) {
     (void)am;
-    int rc = ensure_spinors(c, {B_PHI, B_X});
+    int rc = ensure_spinors(c, {B_X});
     if (rc) return rc;
     if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
     if ((rc = ensure(c, c->b[B_PP], 24 * c->kvol))) return rc;
-    double *dphi = buf<double>(c, B_PHI), *dx = buf<double>(c, B_X), *dchi = buf<double>(c, B_CHI);
+    double *dx = buf<double>(c, B_X), *dchi = buf<double>(c, B_CHI);
     double *dpp = buf<double>(c, B_PP);
-    if ((rc = h2d_spinor(c, dphi, phi))) return rc;
+    const double *dphi = nullptr;
+    if ((rc = in_spinor(c, phi, B_PHI, &dphi))) return rc;
     if ((rc = h2d_spinor(c, dx, x0))) return rc;
     HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(dpp, pp, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
@@ -896,21 +996,23 @@ int chs_hamilton(chs_ctx *c, const double *phi, double *h, double *hg, double *h
                  const double *bnum, const double *bden, int ndiag, double *x, int *itercg) {
     (void)am;
     (void)isweep;
-    int rc = ensure_spinors(c, {B_PHI, B_X});
+    int rc = ensure_spinors(c, {B_X});
     if (rc) return rc;
     if ((rc = ensure(c, c->b[B_CHI], 24 * c->kvol))) return rc;
     if ((rc = ensure(c, c->b[B_PP], 24 * c->kvol))) return rc;
-    double *dphi = buf<double>(c, B_PHI), *dx = buf<double>(c, B_X), *dchi = buf<double>(c, B_CHI);
+    double *dx = buf<double>(c, B_X), *dchi = buf<double>(c, B_CHI);
     double *dpp = buf<double>(c, B_PP);
-    if ((rc = h2d_spinor(c, dphi, phi))) return rc;
+    const double *dphi = nullptr;
+    if ((rc = in_spinor(c, phi, B_PHI, &dphi))) return rc;
     HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(dpp, pp, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     // sums of p^2 and chi^2 (rhmc/cheisdwf.f:546-559), FP64 on the device
     reduce_launch(c, 3 * c->kvol, dpp, nullptr, 2, c->dscal + 8);
     reduce_launch(c, 3 * c->kvol, dchi, nullptr, 2, c->dscal + 9);
     int it = 0;
-    rc = chs_qmrherm_dev(c, dphi, res2, &it, hpin, imass, bnum, bden, ndiag, 0, dchi, dx, nullptr);
-    if (rc) return rc;
+    const int rs = chs_qmrherm_dev(c, dphi, res2, &it, hpin, imass, bnum, bden, ndiag, 0, dchi, dx, nullptr);
+    if (rs != CHS_OK && rs != CHS_ENOCONV) return rs;
+    const std::string msg = c->err;
     reduce_launch(c, c->kferm, dphi, dx, 1, c->dscal + 10);   // Re Phi^dagger X (rhmc/cheisdwf.f:578-580)
     double sums[3];
     HIPCHK(c, hipMemcpyAsync(sums, c->dscal + 8, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -922,7 +1024,8 @@ int chs_hamilton(chs_ctx *c, const double *phi, double *h, double *hg, double *h
     *h = vhg + vhp + vhf;
     *s = vhg + vhf;
     if (itercg) *itercg = it;
-    return CHS_OK;
+    if (rs) c->err = msg;
+    return rs;
 }
 
 }  // extern "C"

@@ -142,7 +142,7 @@ __global__ void k_ms_status(MsGroup gr) {
     if (threadIdx.x != 0) return;
     int nd = 0, it = 0;
     for (int j = 0; j < gr.nsys; j++) {
-        nd += gr.sys[j].lz->done;
+        nd += gr.sys[j].lz->done != 0;   // 1: converged, 2: halted on a non-finite residual
         it = max(it, gr.sys[j].lz->itercg);
     }
     gr.status[0] = nd;

@@ -346,13 +346,16 @@ __global__ void k_lz_alpha(LzState *st, const double *__restrict__ partial, int 
 }
 
 // new beta = |r|; convergence max_k |phimod rho_k| < 2 resid for iter >= 2
-// (rhmc/cheisdwf.f:697-715).
+// (rhmc/cheisdwf.f:697-715).  A non-finite |r|^2 halts the solve with
+// done = 2 (the host reports it; the reference has no such test).
 __device__ __forceinline__ void lz_beta(LzState *st, const double *__restrict__ partial, int np, int iter) {
     const double t = sum_partials(partial, np);
     if (threadIdx.x == 0) {
         st->beta = sqrt(t);
         st->itercg = iter;
-        if (iter >= 2) {
+        if (!isfinite(t)) {
+            st->done = 2;
+        } else if (iter >= 2) {
             double rhomax = 0.0;
             for (int k = 0; k < st->ndiag; k++) {
                 const double a = fabs(st->phimod * st->rho[k]);

@@ -29,6 +29,16 @@ def _p(a):
     return a.ctypes.data_as(_vp)
 
 
+def _check_keep(lat, rc, result):
+    """check(), with the results a solve wrote before stopping at its
+    iteration cap attached to the NoConvergence it raises (.result)."""
+    try:
+        check(lat.h, rc)
+    except _lib.NoConvergence as e:
+        e.result = result
+        raise
+
+
 class Lattice:
     """One lattice geometry and one HIP stream on one GPU (a libchs context).
     Replaces the compile-time parameter(ksize, ksizet, kthird) and /para/ am3,
@@ -69,6 +79,33 @@ class Lattice:
     def synchronize(self):
         check(self.h, lib().chs_synchronize(self.h))
 
+    # ---- device mirrors of read-only host spinors (include/chs.h) ----------
+    def mirror(self, a):
+        """Keep a device copy of the host spinor a (C-contiguous complex128,
+        passed as is to later calls): it is uploaded again only after
+        touch(a).  release(a) before a is freed."""
+        if a.dtype != np.complex128 or not a.flags.c_contiguous or a.size != self.kferm:
+            raise ValueError("a mirrored spinor must be C-contiguous complex128 with kferm elements")
+        check(self.h, lib().chs_mirror_register(self.h, _p(a)))
+        return a
+
+    def touch(self, a):
+        """a (mirrored) was written: its next use uploads it."""
+        check(self.h, lib().chs_mirror_touch(self.h, _p(a)))
+
+    def release(self, a):
+        check(self.h, lib().chs_mirror_release(self.h, _p(a)))
+
+    def transfer_bytes(self):
+        """(host -> device, device -> host) spinor bytes moved so far."""
+        h2d, d2h = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
+        check(self.h, lib().chs_transfer_bytes(self.h, ctypes.byref(h2d), ctypes.byref(d2h)))
+        return h2d.value, d2h.value
+
+    def set_lanczos_cap(self, niterc):
+        """Iteration cap of the multi-shift solves (0: the reference's 5000)."""
+        check(self.h, lib().chs_set_lanczos_cap(self.h, int(niterc)))
+
     def _spinor(self, a):
         a = np.ascontiguousarray(a, dtype=np.complex128)
         if a.size != self.kferm:
@@ -103,7 +140,9 @@ def qmrherm(lat, Phi, res, anum, aden, chi, am=0.0, hpin=0.0, imass=1, iflag=0, 
     """Multi-shift Lanczos solve of (M^dagger M + aden_k) x_k = Phi
     (rhmc/cheisdwf.f:599-745).  Returns (x, itercg): x is /vector/ (a0 Phi +
     sum a_k x_k for iflag=0, a0 Phi for iflag=1).  For iflag=1 the force
-    contributions are added to dSdchi in place."""
+    contributions are added to dSdchi in place.  A solve that reaches the
+    iteration cap (5000) or a non-finite residual raises NoConvergence, whose
+    .result holds (x, itercg) as the reference would return them."""
     Phi = lat._spinor(Phi)
     anum = np.ascontiguousarray(anum, dtype=np.float64)
     aden = np.ascontiguousarray(aden, dtype=np.float64)
@@ -116,8 +155,9 @@ def qmrherm(lat, Phi, res, anum, aden, chi, am=0.0, hpin=0.0, imass=1, iflag=0, 
         if dSdchi is None or dSdchi.dtype != np.float64 or not dSdchi.flags.c_contiguous:
             raise ValueError("iflag=1 needs a C-contiguous float64 dSdchi to accumulate into")
         F = dSdchi
-    check(lat.h, lib().chs_qmrherm(lat.h, _p(Phi), res, ctypes.byref(it), am, hpin, imass, _p(anum), _p(aden),
-                                   len(aden), iflag, _p(lat._field(chi)), _p(x), _p(F) if F is not None else None))
+    rc = lib().chs_qmrherm(lat.h, _p(Phi), res, ctypes.byref(it), am, hpin, imass, _p(anum), _p(aden), len(aden),
+                           iflag, _p(lat._field(chi)), _p(x), _p(F) if F is not None else None)
+    _check_keep(lat, rc, (x.reshape(lat.shape), it.value))
     return x.reshape(lat.shape), it.value
 
 
@@ -143,8 +183,9 @@ def force(lat, Phi, res1, chi, beta, remez2g, am=0.0, hpin=0.0, imass=1):
     F = np.empty((3, lat.kvol))
     x = np.empty_like(Phi)
     its = (ctypes.c_int * 2)()
-    check(lat.h, lib().chs_force(lat.h, _p(Phi), res1, am, hpin, imass, _p(lat._field(chi)), beta, _p(a), _p(ad),
-                                 _p(b), _p(bd), len(ad), _p(F), _p(x), its))
+    rc = lib().chs_force(lat.h, _p(Phi), res1, am, hpin, imass, _p(lat._field(chi)), beta, _p(a), _p(ad), _p(b),
+                         _p(bd), len(ad), _p(F), _p(x), its)
+    _check_keep(lat, rc, (F, x.reshape(lat.shape), (its[0], its[1])))
     return F, x.reshape(lat.shape), (its[0], its[1])
 
 
@@ -200,9 +241,10 @@ def hamilton(lat, Phi, res2, chi, pp, beta, remez2, am=0.0, hpin=0.0, imass=1, i
     out = [ctypes.c_double() for _ in range(4)]
     x = np.empty_like(Phi)
     it = ctypes.c_int(0)
-    check(lat.h, lib().chs_hamilton(lat.h, _p(Phi), *[ctypes.byref(o) for o in out], res2, isweep, am, hpin, imass,
-                                    _p(lat._field(chi)), _p(lat._field(pp)), beta, _p(b), _p(bd), len(bd), _p(x),
-                                    ctypes.byref(it)))
+    rc = lib().chs_hamilton(lat.h, _p(Phi), *[ctypes.byref(o) for o in out], res2, isweep, am, hpin, imass,
+                            _p(lat._field(chi)), _p(lat._field(pp)), beta, _p(b), _p(bd), len(bd), _p(x),
+                            ctypes.byref(it))
+    _check_keep(lat, rc, (tuple(o.value for o in out), x.reshape(lat.shape), it.value))
     return tuple(o.value for o in out), x.reshape(lat.shape), it.value
 
 

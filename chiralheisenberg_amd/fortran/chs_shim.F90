@@ -112,6 +112,16 @@ module chs_binding
       integer(c_int), value :: imass
       real(c_double) :: dsdchi(*)
     end function
+    integer(c_int) function c_mirror_register(c, a) bind(C, name="chs_mirror_register")
+      import :: c_int, c_double_complex, c_ptr
+      type(c_ptr), value :: c
+      complex(c_double_complex) :: a(*)
+    end function
+    integer(c_int) function c_mirror_touch(c, a) bind(C, name="chs_mirror_touch")
+      import :: c_int, c_double_complex, c_ptr
+      type(c_ptr), value :: c
+      complex(c_double_complex) :: a(*)
+    end function
   end interface
 
 contains
@@ -138,6 +148,19 @@ contains
     integer(c_int), intent(in) :: rc
     character(len=*), intent(in) :: where
     if (rc /= 0) call chs_stop(chs_last_error(ctx), where)
+  end subroutine
+
+  ! a multi-shift solve that stopped at its iteration cap: the reference
+  ! carries on silently (rhmc/cheisdwf.f:639); here it carries on with a
+  ! message, and stops on any other error
+  subroutine chs_nonconv(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    if (rc == 3) then
+      write(6, '(a, ": iteration cap reached, result kept")') where
+    else
+      call chs_check(rc, where)
+    end if
   end subroutine
 
   ! create the context on first use, with /para/ am3, ibound of the caller
@@ -184,11 +207,12 @@ subroutine qmrherm(Phi, res, itercg, am, hpin, imass, anum, aden, ndiag, iflag)
   complex*16 Phi(kferm), x
   real*8 chi, anum(0:ndiag), aden(ndiag)
   real(c_double) :: f(kvol, 3)
-  integer(c_int) :: it
+  integer(c_int) :: it, rc
   call chs_ensure(real(am3, c_double), ibound)
   f = real(dSdchi, c_double)
-  call chs_check(c_qmrherm(ctx, Phi, real(res, c_double), it, real(am, c_double), real(hpin, c_double), &
-                           int(imass, c_int), anum, aden, int(ndiag, c_int), int(iflag, c_int), chi, x, f), 'qmrherm')
+  rc = c_qmrherm(ctx, Phi, real(res, c_double), it, real(am, c_double), real(hpin, c_double), &
+                 int(imass, c_int), anum, aden, int(ndiag, c_int), int(iflag, c_int), chi, x, f)
+  call chs_nonconv(rc, 'qmrherm')
   if (iflag == 1) dSdchi = f
   itercg = it
 end subroutine
@@ -239,8 +263,9 @@ subroutine force(Phi, res1, am, hpin, imass)
   real(c_double) :: f(kvol, 3)
   integer(c_int) :: its(2)
   call chs_ensure(real(am3, c_double), ibound)
-  call chs_check(c_force(ctx, Phi, real(res1, c_double), real(am, c_double), real(hpin, c_double), int(imass, c_int), &
-                         chi, real(beta, c_double), anum, aden, bnum, bden, int(ndiagg, c_int), f, X1, its), 'force')
+  call chs_nonconv(c_force(ctx, Phi, real(res1, c_double), real(am, c_double), real(hpin, c_double), &
+                           int(imass, c_int), chi, real(beta, c_double), anum, aden, bnum, bden, int(ndiagg, c_int), &
+                           f, X1, its), 'force')
   dSdchi = f
   ancg = ancg + float(its(1))
   ancgf = ancgf + float(its(2))
@@ -261,12 +286,34 @@ subroutine hamilton(Phi, h, hg, hp, s, res2, isweep, am, hpin, imass)
   integer(c_int) :: it
   call chs_ensure(real(am3, c_double), ibound)
   p = real(pp, c_double)
-  call chs_check(c_hamilton(ctx, Phi, hh, hhg, hhp, ss, real(res2, c_double), int(isweep, c_int), real(am, c_double), &
-                            real(hpin, c_double), int(imass, c_int), chi, p, real(beta, c_double), bnum, bden, &
-                            int(ndiag, c_int), X1, it), 'hamilton')
+  call chs_nonconv(c_hamilton(ctx, Phi, hh, hhg, hhp, ss, real(res2, c_double), int(isweep, c_int), &
+                              real(am, c_double), real(hpin, c_double), int(imass, c_int), chi, p, &
+                              real(beta, c_double), bnum, bden, int(ndiag, c_int), X1, it), 'hamilton')
   h = hh
   hg = hhg
   hp = hhp
   s = ss
   ancgh = ancgh + float(it)
+end subroutine
+
+! ---------------------------------------------------------------------------
+! Device mirror of a read-only spinor the caller passes to many calls (Phi of
+! the trajectory): after `call chs_mirror(Phi)` once, the library keeps Phi on
+! the device and uploads it again only after `call chs_touch(Phi)`, which the
+! caller adds after each write to Phi (the pseudofermion heatbath,
+! rhmc/cheisdwf.f:238-245).  No counterpart in the reference.
+
+subroutine chs_mirror(Phi)
+  use chs_binding
+  common /para/ bbb, am3, ibound
+  complex*16 Phi(kferm)
+  call chs_ensure(real(am3, c_double), ibound)
+  call chs_check(c_mirror_register(ctx, Phi), 'chs_mirror')
+end subroutine
+
+subroutine chs_touch(Phi)
+  use chs_binding
+  complex*16 Phi(kferm)
+  if (.not. c_associated(ctx)) call chs_stop(c_null_ptr, 'chs_touch called before chs_mirror')
+  call chs_check(c_mirror_touch(ctx, Phi), 'chs_touch')
 end subroutine

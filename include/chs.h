@@ -80,7 +80,11 @@ int chs_dslashd(chs_ctx *ctx, double *phi, const double *r, const double *chi,
  * rhmc/cheisdwf.f:599.  The COMMON inputs become arguments: chi = /trial/
  * chit; x = /vector/ x (output: a0 Phi + sum a_k x_k for iflag=0, a0 Phi for
  * iflag=1); dsdchi = /sforce/ dSdchi (accumulated for iflag=1, may be NULL
- * for iflag=0).  anum has ndiag+1 entries, aden ndiag. */
+ * for iflag=0).  anum has ndiag+1 entries, aden ndiag.  Returns CHS_ENOCONV,
+ * with x and dsdchi written as the reference leaves them, when the Lanczos
+ * loop reaches its cap (niterc = 5000, rhmc/cheisdwf.f:603,639, where the
+ * reference returns silently) or |r|^2 turns non-finite; chs_force and
+ * chs_hamilton pass that status on, their results written too. */
 int chs_qmrherm(chs_ctx *ctx, const double *phi, double res, int *itercg,
                 double am, double hpin, int imass, const double *anum,
                 const double *aden, int ndiag, int iflag, const double *chi,
@@ -118,6 +122,30 @@ int chs_qmrherm_prec_dev(chs_ctx *ctx, const double *phi, double res, int *iterc
  * the reference (which computes in FP64 throughout); hamilton, and with it
  * the Metropolis test, stays FP64. */
 int chs_set_force_precision(chs_ctx *ctx, int prec);
+
+/* Iteration cap of every multi-shift solve of the context (the reference's
+ * compile-time niterc = 5000, rhmc/cheisdwf.f:603; 0 restores it).  No
+ * counterpart as a call: for tests of the cap and for callers that want an
+ * earlier bound. */
+int chs_set_lanczos_cap(chs_ctx *ctx, int niterc);
+
+/* ---- device mirrors of host spinors (boundary convention) ----------------
+ * The host-array entry points upload each spinor argument once per call.
+ * A read-only spinor argument that the caller passes again and again
+ * unchanged (Phi: every force and hamilton of a trajectory; R of repeated
+ * dslash calls) can be registered: the library then keeps a device copy,
+ * keyed by the host address, and uploads it only when the array's generation
+ * has moved.  The caller bumps the generation with chs_mirror_touch after
+ * every write to the array (in the reference's driver: after the
+ * pseudofermion heatbath sets Phi, rhmc/cheisdwf.f:238-245).  Arrays written
+ * by a call (x, /vector/, dsdchi) are always transferred.  No counterpart in
+ * the reference. */
+int chs_mirror_register(chs_ctx *ctx, const void *host);   /* kferm complex*16 */
+int chs_mirror_touch(chs_ctx *ctx, const void *host);
+int chs_mirror_release(chs_ctx *ctx, const void *host);
+/* spinor bytes moved host -> device and device -> host by the host-array
+ * entry points since the context was created */
+int chs_transfer_bytes(const chs_ctx *ctx, unsigned long long *h2d, unsigned long long *d2h);
 
 /* H = beta sum chi^2 + 1/2 sum p^2 + Re Phi^dagger (M^dagger M)^{-1/2} Phi.
  * Replaces hamilton(Phi,h,hg,hp,s,res2,isweep,am,hpin,imass),

@@ -137,6 +137,17 @@ class Ref:
             dsdchi[:] = self.dsdchi
         return self.x1.copy(), it.value
 
+    def derivs(self, r, x2, anum, imass, dsdchi):
+        """derivs(R, X2, anum, imass) (rhmc/cheisdwf.f:747-849): /sforce/
+        dSdchi, set from dsdchi first, accumulates the wall bilinears of R
+        and X2.  Returns the updated dSdchi as FP64."""
+        r = np.ascontiguousarray(r, dtype=np.complex128)
+        x2 = np.ascontiguousarray(x2, dtype=np.complex128)
+        self.dsdchi[:] = dsdchi
+        self.lib.derivs_(r.ctypes.data_as(ctypes.c_void_p), x2.ctypes.data_as(ctypes.c_void_p),
+                         ctypes.byref(ctypes.c_double(anum)), ctypes.byref(ctypes.c_int(imass)))
+        return np.array(self.dsdchi, dtype=np.float64)
+
     def congrad(self, phi, res, chi, x0, hpin=0.0, imass=1, am=0.0):
         phi = np.ascontiguousarray(phi, dtype=np.complex128)
         self.chit[:] = chi

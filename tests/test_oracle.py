@@ -119,6 +119,23 @@ def test_dslash_vs_reference_build_direct():
 
 
 @pytest.mark.skipif(not RB.available("fp64"), reason="reference build oracle/_ref absent")
+@pytest.mark.parametrize("imass", [1, 3])
+def test_derivs_vs_reference_build_direct(imass):
+    """orc_derivs alone against the reference's derivs (rhmc/cheisdwf.f:747-849)
+    on unrelated spinors R and X2 (not X2 = M R, so every wall bilinear and
+    its sign is seen), accumulating into a non-zero dSdchi."""
+    rs = np.random.RandomState(40 + imass)
+    shape = (2, 4, RB.LT, RB.L, RB.L, RB.LS)
+    R, X2 = spinor(rs, shape), spinor(rs, shape)
+    F0 = rs.randn(3, RB.KVOL)
+    ref = RB.Ref("fp64")
+    G = RB._run_big_stack(ref.derivs, R, X2, 0.37, imass, F0)
+    F = orc.derivs(LAT12, F0.copy(), R, X2, 0.37, imass)
+    assert np.abs(F - G).max() <= 1e-13 * np.abs(G - F0).max()
+    assert np.abs(G - F0).max() > 1e-3            # the call did change dSdchi
+
+
+@pytest.mark.skipif(not RB.available("fp64"), reason="reference build oracle/_ref absent")
 @pytest.mark.parametrize("ibound", [-1, 1])
 def test_dslash_vs_both_reference_formulations(ibound):
     """The reference holds two formulations of M and M^dagger: dslash/dslashd
