@@ -251,12 +251,17 @@ void orc_derivs(const orc_lat *g, double *F, const double *y, const double *z,
 }
 
 /* ---- multi-shift Lanczos ------------------------------------------------ */
+/* niterc of rhmc/cheisdwf.f:603 (5000); settable so that tests can compare
+ * solves stopped at a small cap */
+static int g_lz_cap = 5000;
+void orc_set_lanczos_cap(int niterc) { g_lz_cap = niterc > 0 ? niterc : 5000; }
+
 int orc_qmrherm_shifts(const orc_lat *g, const double *phi, double res,
                        const double *anum, const double *aden, int ndiag,
                        int iflag, const double *chi, double hpin, int imass,
                        double *x, double *dSdchi, double *xk) {
     const long n = kferm_of(g);
-    const int niterc = 5000;
+    const int niterc = g_lz_cap;
     double resid = sqrt((double)n * res * res);
     double *r = (double *)malloc(sizeof(double) * 2 * n);
     double *q = (double *)malloc(sizeof(double) * 2 * n);

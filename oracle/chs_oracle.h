@@ -45,6 +45,7 @@ int orc_qmrherm(const orc_lat *g, const double *phi, double res,
 
 /* Same solve, but also stores every shifted solution x_k (ndiag*kferm
  * complex, shift-major) in xk when xk is not NULL. */
+void orc_set_lanczos_cap(int niterc);   /* 0: the reference's 5000 */
 int orc_qmrherm_shifts(const orc_lat *g, const double *phi, double res,
                        const double *anum, const double *aden, int ndiag,
                        int iflag, const double *chi, double hpin, int imass,

@@ -62,6 +62,8 @@ def lib():
         _LIB.orc_derivs.argtypes = [L, p, p, p, d, i]
         _LIB.orc_qmrherm_shifts.argtypes = [L, p, d, p, p, i, i, p, d, i, p, p, p]
         _LIB.orc_qmrherm_shifts.restype = i
+        _LIB.orc_set_lanczos_cap.argtypes = [i]
+        _LIB.orc_set_lanczos_cap.restype = None
         _LIB.orc_congrad.argtypes = [L, p, d, p, d, i, p]
         _LIB.orc_congrad.restype = i
         _LIB.orc_cg_iterations.argtypes = [L, p, p, d, i, i, p]
@@ -152,6 +154,11 @@ def qmrherm(lat, phi, res, anum, aden, chi, hpin=0.0, imass=1, iflag=0, dsdchi=N
                                   n, iflag, _ptr(chi), float(hpin), imass, _ptr(x), _ptr(dsdchi),
                                   _ptr(xk) if want_shifts else None)
     return (x, it, xk) if want_shifts else (x, it)
+
+
+def set_lanczos_cap(niterc):
+    """Iteration cap of qmrherm (0: the reference's 5000)."""
+    lib().orc_set_lanczos_cap(int(niterc))
 
 
 def congrad(lat, phi, res, chi, x0=None, hpin=0.0, imass=1):
@@ -287,6 +294,53 @@ def rhmc(lat, params, rng, chi, isweep0, ntraj, remez):
     assert chi.dtype == np.float64 and chi.flags.c_contiguous
     lib().orc_rhmc(ctypes.byref(lat.c), ctypes.byref(par), ctypes.byref(rng.c), _ptr(chi), isweep0, ntraj, outs)
     return [{f: getattr(o, f) for f, _ in OrcTrajOut._fields_} for o in outs]
+
+
+def hmc_trajectory(olat, chi, rng, par):
+    """One trajectory of the HMC sibling's program (mh/cheisdwf_hmc.f:140-330)
+    on the oracle: gauss0 sources in flavour, Dirac, s order, Phi = M^dagger R,
+    the chronological guess X0 = Phi, momenta from two gaussp calls, H0, a
+    leapfrog of random length (one yrun draw per step after its force, end
+    with probability 1/iterl), H1, and the Metropolis draw only when dH < 0
+    (accepted unless the draw exceeds exp(dH)).  chi is replaced in place on
+    acceptance.  Returns dH, dS, accepted, md_steps, H0, H1 and the gauge and
+    momentum parts of both Hamiltonians."""
+    kv = olat.kvol
+    chit = chi.copy()
+    R = np.empty(olat.shape, np.complex128)
+    Rv = R.reshape(2, 4, kv, olat.Ls)
+    for f in range(2):
+        for d in range(4):
+            for s in range(olat.Ls):
+                ps = rng.gauss0(kv)
+                Rv[f, d, :, s] = ps[0] + 1j * ps[1]
+    Phi = dslash(olat, R, chit, 0.0, par["imass"], True)
+    x0 = Phi.copy()
+    pp = np.empty((3, kv))
+    ps = rng.gaussp(kv)
+    pp[0], pp[1] = ps[0], ps[1]
+    pp[2] = rng.gaussp(kv)[0]
+    (H0, hg0, hp0, S0), x0, _ = hmc_hamilton(olat, Phi, x0, par["rescga"], chit, pp, par["beta"], par["imass"])
+    F, x0, _ = hmc_force(olat, Phi, x0, par["rescgg"], chit, par["beta"], par["imass"])
+    pp -= 0.5 * par["dt"] * F
+    steps = 0
+    for step in range(1, par["itermax"] + 1):
+        chit += par["dt"] * pp
+        F, x0, _ = hmc_force(olat, Phi, x0, par["rescgg"], chit, par["beta"], par["imass"])
+        steps = step
+        if rng.rano(1) < 1.0 / par["iterl"]:
+            pp -= 0.5 * par["dt"] * F
+            break
+        pp -= par["dt"] * F
+    (H1, hg1, hp1, S1), x0, _ = hmc_hamilton(olat, Phi, x0, par["rescga"], chit, pp, par["beta"], par["imass"])
+    dH = H0 - H1
+    acc = True
+    if dH < 0:
+        acc = not (rng.rano(1) > np.exp(dH))
+    if acc:
+        chi[...] = chit
+    return dict(dH=dH, dS=S0 - S1, accepted=int(acc), md_steps=steps, H0=H0, H1=H1, hg0=hg0, hp0=hp0, hg1=hg1,
+                hp1=hp1)
 
 
 def set_threads(n):

@@ -110,3 +110,17 @@ def test_shim_force_hamilton_congrad(shim):
     xc, ic = orc.congrad(OLAT, phi, 1e-9, chi, x0)
     assert abs(it.value - ic) <= 3
     assert np.abs(shim.x - xc).max() <= 1e-6 * np.abs(xc).max()
+
+
+@pytest.mark.parametrize("imass", [1, 3])
+def test_shim_derivs_accumulates_into_sforce(shim, imass):
+    rs = np.random.RandomState(4 + imass)
+    R, X2, chi = spinor(rs, SHAPE), spinor(rs, SHAPE), phi_field(rs, KVOL)
+    out = np.empty(SHAPE, np.complex128)
+    # derivs has no lattice arguments: the context exists once any other routine ran
+    shim.lib.dslash_(P(out), P(R), P(chi), byref(0.0), byref(0.0), byref(1, ctypes.c_int))
+    F0 = rs.randn(3, KVOL)
+    shim.F[:] = F0
+    shim.lib.derivs_(P(R), P(X2), byref(0.29), byref(imass, ctypes.c_int))
+    G = orc.derivs(OLAT, F0.copy(), R, X2, 0.29, imass)
+    assert np.abs(shim.F - G).max() <= 1e-13 * np.abs(G - F0).max()

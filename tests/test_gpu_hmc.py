@@ -57,45 +57,6 @@ def test_hmc_force_vs_oracle_other_size():
     lat.close()
 
 
-def oracle_trajectory(olat, chi, rng, par):
-    """The sibling's trajectory on the oracle, same draws as hmc.trajectory."""
-    kv = olat.kvol
-    chit = chi.copy()
-    R = np.empty(olat.shape, np.complex128)
-    Rv = R.reshape(2, 4, kv, olat.Ls)
-    for f in range(2):
-        for d in range(4):
-            for s in range(olat.Ls):
-                ps = rng.gauss0(kv)
-                Rv[f, d, :, s] = ps[0] + 1j * ps[1]
-    Phi = orc.dslash(olat, R, chit, 0.0, par["imass"], True)
-    x0 = Phi.copy()
-    pp = np.empty((3, kv))
-    ps = rng.gaussp(kv)
-    pp[0], pp[1] = ps[0], ps[1]
-    pp[2] = rng.gaussp(kv)[0]
-    (H0, _, _, _), x0, _ = orc.hmc_hamilton(olat, Phi, x0, par["rescga"], chit, pp, par["beta"], par["imass"])
-    F, x0, _ = orc.hmc_force(olat, Phi, x0, par["rescgg"], chit, par["beta"], par["imass"])
-    pp -= 0.5 * par["dt"] * F
-    steps = 0
-    for step in range(1, par["itermax"] + 1):
-        chit += par["dt"] * pp
-        F, x0, _ = orc.hmc_force(olat, Phi, x0, par["rescgg"], chit, par["beta"], par["imass"])
-        steps = step
-        if rng.rano(1) < 1.0 / par["iterl"]:
-            pp -= 0.5 * par["dt"] * F
-            break
-        pp -= par["dt"] * F
-    (H1, _, _, _), x0, _ = orc.hmc_hamilton(olat, Phi, x0, par["rescga"], chit, pp, par["beta"], par["imass"])
-    dH = H0 - H1
-    acc = True
-    if dH < 0:
-        acc = not (rng.rano(1) > np.exp(dH))
-    if acc:
-        chi[...] = chit
-    return dict(dH=dH, accepted=int(acc), md_steps=steps)
-
-
 @pytest.mark.parametrize("imass", [1, 3])
 def test_hmc_trajectories_match_oracle(imass):
     L, Ls, ntraj = 4, 4, 6
@@ -110,7 +71,7 @@ def test_hmc_trajectories_match_oracle(imass):
     olat = orc.Lattice(L, Ls)
     chi_c = chi0.copy()
     orng = orc.Rng(seed)
-    cpu = [oracle_trajectory(olat, chi_c, orng, par) for _ in range(ntraj)]
+    cpu = [orc.hmc_trajectory(olat, chi_c, orng, par) for _ in range(ntraj)]
     print("accepted", [g["accepted"] for g in gpu], "dH", [round(c["dH"], 4) for c in cpu])
     assert [g["md_steps"] for g in gpu] == [c["md_steps"] for c in cpu]
     assert [g["accepted"] for g in gpu] == [c["accepted"] for c in cpu]

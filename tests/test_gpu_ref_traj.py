@@ -1,0 +1,98 @@
+"""Complete runs of the product's drivers against complete runs of the
+reference's own programs (tests/golden/ref_traj_*.npz, written by
+tests/golden/make_ref_traj.py from rhmc/cheisdwf.f and mh/cheisdwf_hmc.f
+compiled unmodified with flang at their compile-time L = 12, Ls = 8, FP64
+kinds).  Same start configuration (the reference's own con files), same run
+parameters, same number of trajectories; then the files both write:
+
+  control    dH and dS per trajectory; accepted (dwf3d) identical
+  fort.11    gauge and momentum action per site after every trajectory
+  fort.12-14 cond, susc, condmod, susctot every fifth trajectory
+  fort.50xx  wall-source correlators of the measurement (dwf3d)
+  con        the final chi and the generator's seed (bit-exact)
+  output     number of acceptances and total MD steps
+
+The seed after the run is a product of the 48-bit multiplier over every
+number drawn, so its equality pins the random stream's consumption -- the
+heatbath's and the momenta's draws, one per MD step, one per Metropolis test
+and one per measurement -- and the shuffle table shared by the two pointer
+streams, including where the never-initialised yrun starts (at 0: flang
+keeps main-program variables in zeroed static storage).
+
+Tolerances: the reference and the GPU form their sums in different orders,
+and at the programs' own stopping residuals (1e-6 for the force, 1e-9 for
+the Hamiltonian) finite-precision CG amplifies that rounding: the sibling's
+force moves by 3.8e-8 relative when its source is perturbed by 1e-15
+relative (the oracle, L=12, Ls=8), and the oracle's and the reference's
+forces differ by 3e-8 on the same inputs.  Over a trajectory that moves dH
+by ~1e-5 for the CG force (measured 2.1e-5 between the oracle and the
+sibling's program), so dH is asserted to 1e-4 for the sibling and to 1e-6 for
+dwf3d (its multi-shift solves are less sensitive), the actions and chi
+correspondingly; decisions, step counts and the seed are exact."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+fm = pytest.importorskip("chiralheisenberg_amd.fermion")
+from chiralheisenberg_amd import checkpoint, hmc, rhmc  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = [c for c in ("rhmc_imass1", "rhmc_imass3", "hmc")
+         if os.path.exists(os.path.join(HERE, "golden", "ref_traj_%s.npz" % c))]
+
+
+def _rows(path):
+    if not os.path.exists(path):
+        return np.zeros((0,))
+    return np.array([[float(t) for t in line.split()] for line in open(path) if line.strip()])
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_driver_matches_reference_program(case, tmp_path):
+    g = np.load(os.path.join(HERE, "golden", "ref_traj_%s.npz" % case))
+    par = {k[4:]: g[k].item() for k in g.files if k.startswith("par_")}
+    for k in ("imass", "iterl", "iter2"):
+        par[k] = int(par[k])
+    n = par["iter2"]
+    d = str(tmp_path)
+    checkpoint.write_con(os.path.join(d, "con"), g["chi0"], float(g["seed0"]))
+    print("case %s: %d trajectories at L=12, Ls=8" % (case, n), flush=True)
+    if case == "hmc":
+        out = hmc.run(d, 12, 8, par=par, iter2=n, iprint=5, icheck=100)
+    else:
+        out = rhmc.run(d, 12, 8, par=par, iter2=n, iprint=5)
+    ctl, ref_ctl = _rows(os.path.join(d, "control")), g["control"]
+    assert ctl.shape == ref_ctl.shape
+    tol = 1e-4 if case == "hmc" else 1e-6
+    dH_err = np.abs(ctl[:, 0] - ref_ctl[:, 0]).max()
+    dS_err = np.abs(ctl[:, 1] - ref_ctl[:, 1]).max()
+    print("max |dH - dH_ref| = %.2e, |dS - dS_ref| = %.2e over dH %s" % (dH_err, dS_err, ref_ctl[:, 0]), flush=True)
+    assert dH_err <= tol
+    assert dS_err <= 10 * tol
+    if case != "hmc":
+        assert np.array_equal(ctl[:, 2], ref_ctl[:, 2])          # accept/reject, trajectory by trajectory
+    assert out["acceptances"] == int(g["naccp"])
+    assert int(round(out["av_traj_length"] * n / par["dt"])) == int(g["itot"])   # total MD steps
+    f11 = _rows(os.path.join(d, "fort.11"))
+    assert np.array_equal(f11[:, 0], g["fort_11"][:, 0])
+    assert np.abs(f11[:, 1:] - g["fort_11"][:, 1:]).max() <= tol * 1e-3 * np.abs(g["fort_11"][:, 1:]).max()
+    for f in ("fort_12", "fort_13", "fort_14"):
+        mine = _rows(os.path.join(d, f.replace("_", ".")))
+        assert mine.shape == g[f].shape
+        assert np.abs(mine - g[f]).max() <= tol * max(np.abs(g[f]).max(), 1e-3)
+    for f in ("fort_5001", "fort_5011", "fort_5002", "fort_5012"):
+        ref = g[f]
+        mine = _rows(os.path.join(d, f.replace("_", ".")))
+        assert mine.shape == ref.shape
+        if ref.size:
+            assert np.array_equal(mine[:, 0], ref[:, 0])
+            # measured 2.2e-6 of the largest entry (imass = 3)
+            assert np.abs(mine[:, 1:] - ref[:, 1:]).max() <= 1e-5 * np.abs(ref[:, 1:]).max()
+    chi1, seed1, _ = checkpoint.read_con(os.path.join(d, "con"), kvol=12 ** 3, precision="fp64")
+    assert seed1 == float(g["seed1"])                            # the random stream, draw for draw
+    chi_err = np.abs(chi1 - g["chi1"]).max()
+    print("max |chi - chi_ref| = %.2e" % chi_err, flush=True)
+    assert chi_err <= tol
