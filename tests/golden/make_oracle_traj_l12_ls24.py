@@ -10,6 +10,12 @@ file is rewritten after every trajectory and holds the first n of them
 (chi_final and seed_final after trajectory n); the test compares those n.
 
     python tests/golden/make_oracle_traj_l12_ls24.py [threads]
+
+With --tight it writes tests/golden/oracle_traj_l12_ls24_tight.npz instead:
+two trajectories from the same start with the solver tolerances tightened to
+rescga = 1e-12 (hamilton, heatbath) and rescgg = 1e-11 (force) and a mean
+length of 4 steps (iterl = 4), where H, dH and the forces are fixed to ~1e-12
+by the mathematics rather than by where two solvers happen to stop.
 """
 import os
 import sys
@@ -30,26 +36,34 @@ PAR = dict(dt=0.05, beta=0.475, hpin=0.0, imass=1, iterl=20, itermax=5000, iprin
            am=0.0)
 
 
+PAR_TIGHT = dict(PAR, rescga=1e-12, rescgg=1e-11, iterl=4)
+NTRAJ_TIGHT = 2
+
+
 def chi0():
     return np.random.RandomState(12).randn(3, L * L * L) / np.sqrt(PAR["beta"])
 
 
 def main():
-    if len(sys.argv) > 1:
-        orc.set_threads(int(sys.argv[1]))
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    tight = "--tight" in sys.argv
+    if args:
+        orc.set_threads(int(args[0]))
+    par, ntraj = (PAR_TIGHT, NTRAJ_TIGHT) if tight else (PAR, NTRAJ)
+    name = "oracle_traj_l12_ls24_tight.npz" if tight else "oracle_traj_l12_ls24.npz"
     sets = {k: remez.load(k) for k in ("remez2", "remez4", "remez2g")}
     olat = orc.Lattice(L, LS)
     chi = chi0()
     rng = orc.Rng(SEED)
     out = []
     t0 = time.time()
-    for i in range(NTRAJ):
-        out += orc.rhmc(olat, PAR, rng, chi, i + 1, 1, sets)
+    for i in range(ntraj):
+        out += orc.rhmc(olat, par, rng, chi, i + 1, 1, sets)
         print("trajectory %d: steps %d accepted %d dH %.6e (%.0f s)" % (i + 1, out[-1]["md_steps"],
                                                                        out[-1]["accepted"], out[-1]["dH"],
                                                                        time.time() - t0), flush=True)
-        keys = ("H0", "H1", "dH", "accepted", "md_steps")
-        path = os.path.join(HERE, "oracle_traj_l12_ls24.npz")
+        keys = ("H0", "H1", "S0", "S1", "dH", "accepted", "md_steps")
+        path = os.path.join(HERE, name)
         np.savez_compressed(path + ".tmp.npz", **{k: np.array([o[k] for o in out]) for k in keys},
                             chi_final=chi, seed_final=np.array(rng.get()))
         os.replace(path + ".tmp.npz", path)
