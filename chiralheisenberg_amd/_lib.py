@@ -90,6 +90,7 @@ def _declare(L):
         "chs_dslash_dev": ([v, v, v, v, d, i, i, i], i),
         "chs_qmrherm_dev": ([v, v, d, ip, d, i, v, v, i, i, v, v, v], i),
         "chs_congrad_dev": ([v, v, d, ip, d, i, v, v], i),
+        "chs_congrad_noalloc_dev": ([v, v, d, ip, d, i, v, v], i),
         "chs_solve_mdagm_dev": ([v, v, v, v, d, i, d, i, d, i, ctypes.POINTER(SolveStats)], i),
         "chs_time_dslash": ([v, i, i, i, i, d, i, ctypes.POINTER(ctypes.c_double)], i),
         "chs_time_cg_stencil": ([v, i, i, i, d, i, ctypes.POINTER(ctypes.c_double)], i),

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -49,6 +50,8 @@ struct chs_ctx {
     int plane_stencil = 1;        // LDS-plane plain FP64 dslash when L^2 <= 1024 (CHS_PLANES=0: off)
     int force_prec = CHS_FP64;    // storage of the force's multi-shift solves (chs_set_force_precision)
     int lz_cap = 5000;            // multi-shift iteration cap (the reference's niterc)
+    void *scratch = nullptr;      // device address of the static scratch on this device
+    std::string scratch_err;
     // Device mirrors of read-only host spinors, keyed by host address: the
     // copy on the device is current while its generation equals the host
     // array's (the caller bumps that with chs_mirror_touch after writing it).
@@ -59,6 +62,45 @@ struct chs_ctx {
 static void nn_destroy(chs_ctx *c);
 
 static thread_local std::string g_create_err;
+
+// ---- static scratch of chs_congrad_noalloc_dev --------------------------------
+// Device memory that exists for the life of the process on every GPU the
+// library is loaded on, so the entry point that uses it calls no allocator.
+// In host code `scratch` names a host-side stand-in, not device memory: its
+// device address comes from hipGetSymbolAddress, once per device, and only
+// that address is ever passed to kernels or copies.
+constexpr size_t kNoAllocPartialOff = 4096;          // after the CG state
+constexpr size_t kNoAllocSpinorOff = 64 * 1024;      // after the partial sums (up to 7680 doubles)
+constexpr size_t kNoAllocBytes = kNoAllocSpinorOff + 3 * 16 * (size_t)CHS_NOALLOC_MAX_KFERM;
+static_assert(sizeof(chs::CgState) <= kNoAllocPartialOff, "CG state does not fit its scratch slot");
+static_assert((kNoAllocSpinorOff - kNoAllocPartialOff) / sizeof(double) >=
+                  (CHS_NOALLOC_MAX_KFERM + chs::kBlock - 1) / chs::kBlock + 8,
+              "partial sums do not fit their scratch slot");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[64];            // device address of `scratch`, per device
+static std::string g_scratch_err[64];
+
+// With `device` current: the device address of the scratch array, looked up
+// on the first call for that device.
+static void *scratch_address(int device, std::string &err) {
+    if (device < 0 || device >= 64) {
+        err = "device index outside 0..63";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device] && g_scratch_err[device].empty()) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+        else
+            g_scratch_dev[device] = ptr;
+    }
+    err = g_scratch_err[device];
+    return g_scratch_dev[device];
+}
 
 namespace {
 
@@ -342,6 +384,7 @@ chs_ctx *chs_ctx_create(int L, int Lt, int Ls, int ibound, double am3, int devic
         chs_ctx_destroy(c);
         return nullptr;
     }
+    c->scratch = scratch_address(device, c->scratch_err);
     if (ensure_partials(c) != CHS_OK) {
         g_create_err = c->err;
         chs_ctx_destroy(c);
@@ -694,12 +737,18 @@ int chs_qmrherm(chs_ctx *c, const double *phi, double res, int *itercg, double a
 }
 
 // ---- CGNR with the reference recurrences --------------------------------------
-int chs_congrad_dev(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass,
-                    const double *chi, double *x) {
+}  // extern "C"
+
+namespace {
+
+// CGNR (rhmc/cheisdwf.f:1064-1157) with its work spinors p, r, x1 = M p, the
+// stencils' partial sums and the CG state in the buffers the caller passes:
+// the context's (chs_congrad_dev) or the library's static scratch
+// (chs_congrad_noalloc_dev).  partial holds at least nblocks(kferm) doubles.
+int congrad_impl(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass, const double *chi,
+                 double *x, double2 *p, double2 *r, double2 *x1, double *partial, CgState *cg) {
     const long n = c->kferm;
-    int rc = ensure_spinors(c, {B_P, B_R, B_T1});
-    if (rc) return rc;
-    double2 *p = buf<double2>(c, B_P), *r = buf<double2>(c, B_R), *x1 = buf<double2>(c, B_T1);
+    int rc = CHS_OK;
     CgState h;
     memset(&h, 0, sizeof h);
     h.alpha = 1.0;
@@ -707,11 +756,11 @@ int chs_congrad_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
     h.resid = (double)n * res * res;
     h.maxiter = (int)(c->Ls * c->kvol);
     h.aslot = kAlphaEmpty;
-    HIPCHK(c, hipMemcpyAsync(c->cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cg, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(p, x, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(r, phi, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     const int np = (int)stencil_grid(c->n5);
-    const int *skip = &c->cg->halt;
+    const int *skip = &cg->halt;
     const int chunk = chunk_for(c, 11.0 * 16.0 * n);
     int halt = 0, nx = 1;
     while (nx <= h.maxiter && !halt) {
@@ -721,25 +770,25 @@ int chs_congrad_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
                 dslash_launch<F64, F64, false, EPI_STORE>(c, x1, p, chi, hpin, imass,
                                                          epi(nullptr, nullptr, nullptr, nullptr, nullptr, skip));
                 dslash_launch<F64, F64, true, EPI_CG>(c, nullptr, x1, chi, hpin, imass,
-                                                     epi(c->partial, &c->cg->alpha, p, x, r, skip, 1));
+                                                     epi(partial, &cg->alpha, p, x, r, skip, 1));
             } else {
                 // alpha = alphan / |M p|^2 (alphan = |r|^2 = rnorm2) inside the
                 // fused stencil, from the M p launch's partials
                 dslash_launch<F64, F64, false, EPI_NORM>(c, x1, p, chi, hpin, imass,
-                                                        epi(c->partial, nullptr, nullptr, nullptr, nullptr, skip));
+                                                        epi(partial, nullptr, nullptr, nullptr, nullptr, skip));
                 dslash_launch<F64, F64, true, EPI_CGW>(c, nullptr, x1, chi, hpin, imass,
-                                                      epi(c->partial, nullptr, p, x, r, skip, 0, c->cg));
+                                                      epi(partial, nullptr, p, x, r, skip, 0, cg));
             }
-            hipLaunchKernelGGL(k_cg_beta_ref, dim3(1), dim3(kBlock), 0, c->stream, c->cg, (const double *)c->partial,
+            hipLaunchKernelGGL(k_cg_beta_ref, dim3(1), dim3(kBlock), 0, c->stream, cg, (const double *)partial,
                                np, nx);
             hipLaunchKernelGGL((k_cg_pupdate<F64, F64>), dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, p,
-                               (const double2 *)r, (const CgState *)c->cg);
+                               (const double2 *)r, (const CgState *)cg);
         }
         HIPCHK(c, hipGetLastError());
-        if ((rc = poll_int(c, &c->cg->halt, &halt))) return rc;
+        if ((rc = poll_int(c, &cg->halt, &halt))) return rc;
     }
     CgState fin;
-    HIPCHK(c, hipMemcpyAsync(&fin, c->cg, sizeof fin, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&fin, cg, sizeof fin, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (itercg) *itercg = fin.iter;
     if (fin.aerr) return fail(c, CHS_EHIP, "congrad: alpha not published within the wait bound (iteration %d)",
@@ -748,6 +797,40 @@ int chs_congrad_dev(chs_ctx *c, const double *phi, double res, int *itercg, doub
         return fail(c, CHS_ENOCONV, "# iterations of congrad exceeds niterc (%d)", fin.iter);
     return CHS_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int chs_congrad_dev(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass,
+                    const double *chi, double *x) {
+    int rc = ensure_spinors(c, {B_P, B_R, B_T1});
+    if (rc) return rc;
+    return congrad_impl(c, phi, res, itercg, hpin, imass, chi, x, buf<double2>(c, B_P), buf<double2>(c, B_R),
+                        buf<double2>(c, B_T1), c->partial, c->cg);
+}
+
+// The same CGNR with every buffer it writes besides x in the static scratch
+// array (layout: CgState at 0, the partial sums at kNoAllocPartialOff, then
+// p, r and M p, 16*kferm bytes each).  Nothing is allocated and nothing is
+// freed; the scratch keeps the last call's contents, and every field the
+// solve reads is written by it first (the CG state by one host-to-device
+// copy, p and r from x and phi, the partials by the launch before the one
+// that reads them).
+int chs_congrad_noalloc_dev(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass,
+                            const double *chi, double *x) {
+    if (c->kferm > CHS_NOALLOC_MAX_KFERM)
+        return fail(c, CHS_EINVAL,
+                    "congrad without allocation: kferm = %ld exceeds the static scratch's %ld (%zu bytes)", c->kferm,
+                    (long)CHS_NOALLOC_MAX_KFERM, kNoAllocBytes);
+    if (!c->scratch) return fail(c, CHS_EHIP, "congrad without allocation: %s", c->scratch_err.c_str());
+    unsigned char *base = (unsigned char *)c->scratch;
+    double2 *p = (double2 *)(base + kNoAllocSpinorOff);
+    double2 *r = p + c->kferm, *x1 = r + c->kferm;
+    return congrad_impl(c, phi, res, itercg, hpin, imass, chi, x, p, r, x1, (double *)(base + kNoAllocPartialOff),
+                        (CgState *)base);
+}
+
 
 // fermion (rhmc/cheisdwf.f:897-1058): five wall sources at t-slices
 // mod(ittt0 + k (Lt/5), Lt) + 1, Dirac 3 and 4, flavour 1, 1/L^2 at s = 1;

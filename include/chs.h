@@ -225,6 +225,21 @@ int chs_qmrherm_dev(chs_ctx *ctx, const double *phi, double res, int *itercg,
 int chs_congrad_dev(chs_ctx *ctx, const double *phi, double res, int *itercg,
                     double hpin, int imass, const double *chi, double *x);
 
+/* chs_congrad_dev that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code that must not
+ * call hipMalloc, such as tight serving loops or harnesses that forbid it).
+ * The work spinors p, r and M p, the partial sums and the CG state live in a
+ * static device array of the library, allocated with the code object on
+ * every GPU and held until the process ends: kferm <= CHS_NOALLOC_MAX_KFERM
+ * (2^20 complex numbers, e.g. L = 16, Ls = 32; 48 MiB + 64 KiB of scratch),
+ * larger lattices are rejected with CHS_EINVAL.  Same iterates, results and
+ * status as chs_congrad_dev (bit for bit).  The array is shared by every
+ * call on one GPU: calls that overlap on one GPU (two host threads, two
+ * contexts' streams) race and must be serialised by the caller. */
+#define CHS_NOALLOC_MAX_KFERM (1L << 20)
+int chs_congrad_noalloc_dev(chs_ctx *ctx, const double *phi, double res, int *itercg,
+                            double hpin, int imass, const double *chi, double *x);
+
 /* Statistics of one M^dagger M solve. */
 typedef struct {
     int iterations;        /* total inner (Krylov) iterations */
