@@ -1295,6 +1295,7 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
                               std::sqrt(rn2 / bnorm2));
                 break;
             }
+            bool restart = false;
             if (rn2 < best_rn2) {
                 best_rn2 = rn2;
                 stall = 0;
@@ -1304,6 +1305,13 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
                               "reliable updates",
                               std::sqrt(rn2 / bnorm2), h.iter, std::sqrt(best_rn2 / bnorm2), kStall);
                 break;
+            } else {
+                // the true residual did not improve since the last update: the
+                // kept direction carries more rounding than Krylov information
+                // (BF16 storage at L=16, Ls=32, beta=0.45 reaches this), so CG
+                // restarts from p = r
+                restart = true;
+                st.restarts++;
             }
             // the iterated residual had reached the stopping region but the
             // true one had not: tighten the trigger
@@ -1315,7 +1323,7 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
             // first made orthogonal to the new residual (p -= (<r,p>/|r|^2) r,
             // as the reliable-update CG of QUDA does): p_{k+1} = r + beta p_k'
             // from p_k into the other buffer; nothing is pending
-            h.beta = rn2 / h.rnorm2;
+            h.beta = restart ? 0.0 : rn2 / h.rnorm2;
             const double cr = rp / rn2;   // in p's stored units
             if constexpr (S::scaled) {
                 const double sig_new = std::sqrt(rn2 / (double)n);

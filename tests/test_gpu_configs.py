@@ -1,0 +1,113 @@
+"""The workloads the north star names, at their full sizes, on the GPU.
+
+Config 1 -- plain RHMC at L = 8, Ls = 8, beta = 0.475, M5 = 1, FP64: one
+  trajectory of random length (dt = 0.05, iterl = 20: mean 20 steps) with
+  every fermion routine on the GPU against the oracle's trajectory loop
+  (orc_rhmc, itself pinned to the reference's own program by
+  tests/test_ref_traj_oracle.py) on the same inputs and generator, hpin 0 and
+  0.5; and 100 applications of dslash and dslashd (50 rounds of the power
+  iteration v <- M^dagger M v / |M^dagger M v| on a seeded Gaussian spinor)
+  against the oracle's.
+Config 3 -- M^dagger M solves to an FP64 true relative residual of 1e-10 at
+  L = 16 and 24, Ls = 32, beta in {0.45, 0.475, 0.5}, in every reduced-
+  precision form (FP32, scaled binary16 and BF16 storage with FP64 reliable
+  updates): the true residual is recomputed here with the oracle's operator
+  (rhmc/cheisdwf.f:2634-3501 restated), and each solution is compared with the
+  plain FP64 GPU solve of the same system (the error a 1e-10 residual allows
+  on these condition numbers, stated per case)."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import phi_field, spinor
+
+pytestmark = pytest.mark.gpu
+
+fm = pytest.importorskip("chiralheisenberg_amd.fermion")
+from chiralheisenberg_amd import remez, rhmc  # noqa: E402
+
+
+@pytest.mark.parametrize("hpin", [0.0, 0.5])
+def test_config1_trajectory_l8_ls8(hpin):
+    L, Ls = 8, 8
+    par = dict(dt=0.05, beta=0.475, am=0.0, hpin=hpin, imass=1, iterl=20, itermax=1000, iprint=5, rescga=1e-9,
+               rescgg=1e-6)
+    sets = {k: remez.load(k) for k in ("remez2", "remez4", "remez2g")}
+    chi0 = np.random.RandomState(8).randn(3, L ** 3) / np.sqrt(par["beta"])
+    seed = 693764523745.0
+    lat = fm.Lattice(L, Ls)
+    chi_g = chi0.copy()
+    rng = rhmc.RandomStream(seed)
+    g = rhmc.trajectory(lat, chi_g, rng, 1, par, sets)
+    lat.close()
+    olat = orc.Lattice(L, Ls)
+    chi_c = chi0.copy()
+    orng = orc.Rng(seed)
+    c = orc.rhmc(olat, par, orng, chi_c, 1, 1, sets)[0]
+    print("config 1 (hpin %g): %d MD steps, dH gpu %.10f oracle %.10f" % (hpin, g["md_steps"], g["dH"], c["dH"]),
+          flush=True)
+    assert g["md_steps"] == c["md_steps"]
+    assert g["accepted"] == c["accepted"]
+    for k in ("H0", "H1"):
+        assert abs(g[k] - c[k]) <= 1e-10 * abs(c[k])
+    assert abs(g["dH"] - c["dH"]) <= 1e-7
+    assert np.abs(chi_g - chi_c).max() <= 1e-8 * np.abs(chi_c).max()
+    assert rng.seed() == orng.get()
+
+
+def test_config1_hundred_dslash_dslashd():
+    L, Ls = 8, 8
+    lat, olat = fm.Lattice(L, Ls), orc.Lattice(L, Ls)
+    rs = np.random.RandomState(81)
+    v, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    vo = v.copy()
+    for _ in range(50):
+        v = fm.dslashd(lat, fm.dslash(lat, v, chi, 0.0, 0.0, 1), chi, 0.0, 0.0, 1)
+        v /= np.linalg.norm(v)
+        vo = orc.dslash(olat, orc.dslash(olat, vo, chi, 0.0, 1, False), chi, 0.0, 1, True)
+        vo /= np.linalg.norm(vo)
+    lat.close()
+    assert np.abs(v - vo).max() <= 1e-12 * np.abs(vo).max()
+
+
+_FP64 = {}
+
+
+def _config3_system(L, beta):
+    """The bench's system: phi_a ~ N(0, 1/beta) and the unit complex Gaussian
+    source drawn by the reference's generator (seed 693764523745), the inputs
+    on which BF16 storage once diverged at L=16, beta=0.45."""
+    import bench
+    lat = fm.Lattice(L, 32)
+    chi, b = bench.make_inputs(lat, beta)
+    return lat, b, chi
+
+
+def _solve(lat, b, chi, prec):
+    bd = fm.spinor_dev(lat, b)
+    xd = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+    st = fm.solve_mdagm(lat, bd, xd, fm.DeviceArray.from_numpy(lat, chi), 1e-10, prec)
+    return fm.to_host(lat, xd), st
+
+
+@pytest.mark.parametrize("L", [16, 24])
+@pytest.mark.parametrize("beta", [0.45, 0.475, 0.5])
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "bf16"])
+def test_config3_mixed_precision_solves(L, beta, prec):
+    lat, b, chi = _config3_system(L, beta)
+    if (L, beta) not in _FP64:
+        _FP64[(L, beta)] = _solve(lat, b, chi, "fp64")
+    x64, st64 = _FP64[(L, beta)]
+    print("config 3 L=%d beta=%g %s" % (L, beta, prec), flush=True)
+    x, st = _solve(lat, b, chi, prec)
+    lat.close()
+    olat = orc.Lattice(L, 32)
+    r = b - orc.dslash(olat, orc.dslash(olat, x, chi, 0.0, 1, False), chi, 0.0, 1, True)
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    err = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+    print("  %s: %d iterations (FP64 %d), %d reliable updates, %d restarts, true resid %.2e, |x - x64|/|x64| "
+          "%.2e, %.1f ms" % (prec, st["iterations"], st64["iterations"], st["reliable_updates"], st["restarts"],
+                             rel, err, 1e3 * st["seconds"]), flush=True)
+    assert rel <= 1.05e-10
+    assert abs(st["true_resid"] - rel) <= 1e-3 * rel
+    assert err <= 1e-7
