@@ -65,30 +65,97 @@ def measured_traffic(name, L, Ls):
     return rec["traffic_bytes"]
 
 
-def cpu_baseline(L, Ls, beta, fp64_iters, budget_s=20.0):
-    """Bounded sample of the CPU oracle (C/OpenMP, FP64, all host cores):
-    a few CGNR iterations of the same system, scaled by the FP64 GPU solve's
-    iteration count to the metric's unit (ms per solve)."""
+def host_cpu():
+    """(model name, logical CPUs of the host, CPUs this process may use)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return model, os.cpu_count(), allowed
+
+
+def cpu_baseline(L, Ls, beta, gpu_fp64, full=False):
+    """The CPU oracle (C/OpenMP, FP64; the reference's algorithm restated) on
+    the host cores, threads bound one per core (OMP_PROC_BIND=close,
+    OMP_PLACES=cores, as the reference's own batch script runs it), timed
+    without extrapolation on the same workload as the GPU:
+      - one complete CG solve of M^dagger M x = b at L, Ls (the bench's system,
+        from x = 0, stopping rule |r|^2 < kferm res^2 with res = 1e-10, i.e. a
+        relative residual of 1e-10 for a unit Gaussian source): `value`;
+      - 100 dslash and 100 dslashd applications at L, Ls;
+      - with full=True (bench.py --cpu-full; minutes on the CPU) also the
+        config-2 multi-shift solve (L = 12, Ls = 24, remez4, res 1e-10) and
+        force (remez2g, res 1e-6).
+    Threads: OMP_NUM_THREADS when the environment sets it (the host's share
+    for one GPU), else every CPU this process may use."""
+    model, ncpu, allowed = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or allowed)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = min(threads, 16)
+    from chiralheisenberg_amd import remez
     orc.set_threads(threads)
+    print("bench: cpu baseline on %d threads (%s; %d logical CPUs, %d allowed)" % (threads, model, ncpu, allowed),
+          file=sys.stderr, flush=True)
     olat = orc.Lattice(L, Ls)
-    rs = np.random.RandomState(3)
-    chi = rs.randn(3, olat.kvol) / np.sqrt(beta)
-    b = (rs.randn(*olat.shape) + 1j * rs.randn(*olat.shape)) / np.sqrt(2)
+    kvol = olat.kvol
+    V, C = olat.kferm * 16, 3 * kvol * 8
+    chi, b = make_inputs(olat, beta)     # the same system as the GPU's (the oracle lattice has its shape)
+    out = {}
     t0 = time.perf_counter()
-    orc.cg_iterations(olat, b, chi, 1)
-    t1 = time.perf_counter() - t0
-    n = max(1, min(20, int(budget_s / max(t1, 1e-3))))
+    for _ in range(100):
+        orc.dslash(olat, b, chi, 0.0, 1, False)
+    t1 = time.perf_counter()
+    for _ in range(100):
+        orc.dslash(olat, b, chi, 0.0, 1, True)
+    t2 = time.perf_counter()
+    out["dslash_ms"] = (t1 - t0) * 10.0
+    out["dslashd_ms"] = (t2 - t1) * 10.0
+    out["dslash_GB_per_s"] = (2 * V + C) / ((t1 - t0) / 100) / 1e9
+    print("bench: cpu dslash %.1f ms, dslashd %.1f ms" % (out["dslash_ms"], out["dslashd_ms"]), file=sys.stderr,
+          flush=True)
     t0 = time.perf_counter()
-    orc.cg_iterations(olat, b, chi, n)
-    per_it = (time.perf_counter() - t0) / n
-    return {"value": per_it * fp64_iters * 1e3, "unit": "ms per solve (extrapolated)", "cores": threads,
-            "kind": "port",
-            "sample": "%d CGNR iterations of the FP64 oracle at L=%d Ls=%d (%.1f ms/iteration) x %d iterations of "
-                      "the FP64 GPU solve" % (n, L, Ls, per_it * 1e3, fp64_iters)}
+    x, its = orc.congrad(olat, b, 1e-10, chi)
+    cg_s = time.perf_counter() - t0
+    r = b - orc.dslash(olat, orc.dslash(olat, x, chi, 0.0, 1, False), chi, 0.0, 1, True)
+    rel = float(np.linalg.norm(r) / np.linalg.norm(b))
+    print("bench: cpu CG solve %.1f s, %d iterations, true resid %.2e" % (cg_s, its, rel), file=sys.stderr,
+          flush=True)
+    out["cg_solve"] = {"s": cg_s, "iterations": int(its), "true_resid": rel}
+    if not full:
+        return baseline_line(out, L, Ls, its, rel, threads, model, ncpu, cg_s, gpu_fp64)
+    # config 2: L = 12, Ls = 24
+    l2 = orc.Lattice(12, 24)
+    rs = np.random.RandomState(2)
+    chi2 = rs.randn(3, l2.kvol) / np.sqrt(beta)
+    phi2 = (rs.randn(*l2.shape) + 1j * rs.randn(*l2.shape)) / np.sqrt(2)
+    a4, ad4, _, _ = remez.load("remez4")
+    t0 = time.perf_counter()
+    _, it_ms = orc.qmrherm(l2, phi2, 1e-10, a4, ad4, chi2)
+    ms_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, _, it_f = orc.force(l2, phi2, 1e-6, chi2, beta, remez.load("remez2g"))
+    f_s = time.perf_counter() - t0
+    print("bench: cpu config-2 multi-shift %.1f s (%d its), force %.1f s" % (ms_s, it_ms, f_s), file=sys.stderr,
+          flush=True)
+    out["config2"] = {"multishift_remez4_s": ms_s, "multishift_iterations": int(it_ms), "force_s": f_s,
+                      "force_iterations": list(it_f) if hasattr(it_f, "__len__") else it_f}
+    return baseline_line(out, L, Ls, its, rel, threads, model, ncpu, cg_s, gpu_fp64)
+
+
+def baseline_line(out, L, Ls, its, rel, threads, model, ncpu, cg_s, gpu_fp64):
+    return dict({"value": cg_s * 1e3, "unit": "ms per solve", "cores": threads, "kind": "port",
+                 "cpu": model, "host_logical_cpus": ncpu,
+                 "sample": "one complete FP64 CG solve of the bench's system at L=%d Ls=%d (%d iterations, "
+                           "true relative residual %.2e) by the CPU oracle, %d OpenMP threads bound to cores; "
+                           "GPU FP64 form on the same system: %.1f ms" % (L, Ls, its, rel, threads, gpu_fp64),
+                 "speedup_gpu_fp64_vs_cpu": cg_s * 1e3 / gpu_fp64}, **out)
 
 
 BATCH_MIXES = {"full": ((8, 8), (12, 16), (16, 32), (24, 32)), "small": ((8, 8), (12, 8), (12, 16), (16, 16))}
@@ -262,6 +329,8 @@ def main():
     ap.add_argument("--form", default="fp16", choices=["fp64", "fp32", "bf16", "fp16"])
     ap.add_argument("--delta", type=float, default=0.0, help="reliable-update threshold (0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU baseline also times the config-2 multi-shift solve and force (minutes)")
     ap.add_argument("--counters", action="store_true",
                     help="only the roofline kernels and calibration copies (for the counter runs)")
     ap.add_argument("--batch", action="store_true", help="only the batched multi-shift workload (one JSON line)")
@@ -432,7 +501,7 @@ def main():
         out["force"] = force_bench(fm, lat, chi, args.beta)
         print("bench: force %s" % json.dumps(out["force"]), file=sys.stderr, flush=True)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.L, args.Ls, args.beta, st64["iterations"])
+        out["cpu_baseline"] = cpu_baseline(args.L, args.Ls, args.beta, st64["seconds"] * 1e3, args.cpu_full)
     else:
         out["cpu_baseline"] = None
     lat.close()
