@@ -327,16 +327,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
 // ---- plane stencil (FP64 dslash without epilogue, L^2 <= 1024) -----------
 // One workgroup per (flavour, s, t) xy-plane, one thread per site of it: the
 // plane's four spin components are staged in LDS (4 L^2 elements, 64 KB at
-// L = 32) and the x and y neighbours are read from there; the t and s
-// neighbours and the wall operands are global loads issued before the
-// barrier.  Terms are added in the order of stencil().  With its inputs in
-// HBM this form of the plain FP64 dslash is ~10% faster than the one-site
-// kernel (31.3 against 34.7-35.3 us at L=24, Ls=32; equal when the inputs sit
-// in the Infinity Cache); the fused-epilogue stencils keep the one-site form.
+// L = 32) and the x and y neighbours are read from there.  The t neighbours
+// are loaded with the centre and their two hops summed into their own
+// accumulator before the barrier, the s neighbours and the wall operands are
+// loaded after it: that keeps 72 VGPRs live (no spill at the 7 waves/SIMD
+// the attribute asks for), so three 576-thread workgroups share a CU at
+// L = 24 instead of two.  Measured at L=24, Ls=32 with inputs from HBM:
+// 28.9 us against 31.8 us with all loads before the barrier (94 VGPRs) and
+// 35.1 us for the one-site kernel; forcing 8 waves/SIMD spills and takes 50
+// us.  The t-hop sum joins after the x and y hops, so the result differs from
+// stencil()'s term order by rounding only (<= 1 ulp of the largest term).
+// The fused-epilogue stencils keep the one-site form.
 template <class S, bool DAG>
-__global__ __launch_bounds__(1024) void k_dsl_plane(Lat lat, typename S::st *__restrict__ out,
-                                                    const typename S::st *__restrict__ in,
-                                                    const double *__restrict__ chi, double hpin, int imass) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(7))) void k_dsl_plane(
+    Lat lat, typename S::st *__restrict__ out, const typename S::st *__restrict__ in, const double *__restrict__ chi,
+    double hpin, int imass) {
     typedef typename S::rt T;
     typedef typename S::st st;
     extern __shared__ unsigned char smem[];
@@ -374,12 +379,6 @@ __global__ __launch_bounds__(1024) void k_dsl_plane(Lat lat, typename S::st *__r
     constexpr int dp = DAG ? 2 : 0;
     const bool last = (sc + 1 == Ls), first = (sc == 0);
     const unsigned gsp = last ? g : g + kvol, gsm = first ? g : g - kvol;
-    cx<T> up[2], dw[2];
-#pragma unroll
-    for (int dd = 0; dd < 2; dd++) {
-        up[dd] = S::ld(pl[dn + dd], gsp);
-        dw[dd] = S::ld(pl[dp + dd], gsm);
-    }
     if (act) {
 #pragma unroll
         for (int d = 0; d < 4; d++) {
@@ -388,14 +387,26 @@ __global__ __launch_bounds__(1024) void k_dsl_plane(Lat lat, typename S::st *__r
             sh[d * L2 + xy] = w;
         }
     }
-    __syncthreads();
-    if (!act) return;
-#pragma unroll
-    for (int d = 0; d < 4; d++) o[d] = cscale(dg, c[d]);
     const T mh = T(-0.5);
     constexpr int XF = DAG ? -1 : 0, XB = DAG ? 0 : -1;
     constexpr int YF = DAG ? -2 : 1, YB = DAG ? 1 : -2;
     constexpr int TF = DAG ? -3 : 2, TB = DAG ? 2 : -3;
+    // antiperiodic in t across it = Lt-1 <-> 0 (rhmc/cheisdwf.f:2831-2876)
+    cx<T> ot[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) ot[d] = {T(0), T(0)};
+    hop_vals<T, TF>(ot, tf[0], tf[1], tf[2], tf[3], (lat.apbc && it + 1 == Lt) ? T(0.5) : mh);
+    hop_vals<T, TB>(ot, tb[0], tb[1], tb[2], tb[3], (lat.apbc && it == 0) ? T(0.5) : mh);
+    __syncthreads();
+    if (!act) return;
+    cx<T> up[2], dw[2];
+#pragma unroll
+    for (int dd = 0; dd < 2; dd++) {
+        up[dd] = S::ld(pl[dn + dd], gsp);
+        dw[dd] = S::ld(pl[dp + dd], gsm);
+    }
+#pragma unroll
+    for (int d = 0; d < 4; d++) o[d] = cscale(dg, c[d]);
     const unsigned xf = (ix + 1 == L) ? xy - (L - 1) : xy + 1;
     const unsigned xb = (ix == 0) ? xy + (L - 1) : xy - 1;
     const unsigned yf = (iy + 1 == L) ? xy - (L - 1) * L : xy + L;
@@ -404,9 +415,8 @@ __global__ __launch_bounds__(1024) void k_dsl_plane(Lat lat, typename S::st *__r
     hop_vals<T, XB>(o, S::ld(sh, xb), S::ld(sh + L2, xb), S::ld(sh + 2 * L2, xb), S::ld(sh + 3 * L2, xb), mh);
     hop_vals<T, YF>(o, S::ld(sh, yf), S::ld(sh + L2, yf), S::ld(sh + 2 * L2, yf), S::ld(sh + 3 * L2, yf), mh);
     hop_vals<T, YB>(o, S::ld(sh, yb), S::ld(sh + L2, yb), S::ld(sh + 2 * L2, yb), S::ld(sh + 3 * L2, yb), mh);
-    // antiperiodic in t across it = Lt-1 <-> 0 (rhmc/cheisdwf.f:2831-2876)
-    hop_vals<T, TF>(o, tf[0], tf[1], tf[2], tf[3], (lat.apbc && it + 1 == Lt) ? T(0.5) : mh);
-    hop_vals<T, TB>(o, tb[0], tb[1], tb[2], tb[3], (lat.apbc && it == 0) ? T(0.5) : mh);
+#pragma unroll
+    for (int d = 0; d < 4; d++) o[d] = cadd(o[d], ot[d]);
     const cx<T> zero = {T(0), T(0)};
 #pragma unroll
     for (int dd = 0; dd < 2; dd++) {

@@ -178,12 +178,14 @@ def test_pair_stencil_equals_one_site(prec, L, Ls, Lt, monkeypatch):
             assert np.array_equal(outs[0], outs[1]), (prec, imass, dagger, np.abs(outs[0] - outs[1]).max())
 
 
-@pytest.mark.parametrize("L,Ls,Lt", [(2, 2, 2), (5, 3, 4), (8, 8, 8), (24, 4, 6)])
+@pytest.mark.parametrize("L,Ls,Lt", [(2, 2, 2), (5, 3, 4), (8, 8, 8), (24, 4, 6), (32, 4, 4)])
 def test_plane_stencil_matches_one_site(L, Ls, Lt, monkeypatch):
     """The plain FP64 operator stages xy-planes in LDS (L^2 <= 1024); with
-    CHS_PLANES=0 it runs the one-site stencil.  Both add the same terms in the
-    same order: outputs agree to rounding (1e-15 relative), with the wall
-    terms (both imass, hpin at site 0) and the antiperiodic t boundary."""
+    CHS_PLANES=0 it runs the one-site stencil.  The plane kernel sums the two
+    t hops apart and adds them after the x and y hops: outputs agree to
+    rounding (1e-15 relative), with the wall terms (both imass, hpin at site 0)
+    and the antiperiodic t boundary, up to L = 32 (1024-thread workgroups,
+    64 KB of LDS)."""
     monkeypatch.setenv("CHS_PLANES", "0")
     one = fm.Lattice(L, Ls, Lt)
     monkeypatch.setenv("CHS_PLANES", "1")
