@@ -16,11 +16,11 @@ oracle.hmc_trajectory) to complete runs of the reference's own programs
 2. The oracle's drivers run the same trajectories from the same con files
    and reproduce the programs' dH, decisions and actions (and, for the HMC
    sibling, all five trajectories: final chi and seed).  At L = 12, Ls = 8
-   the oracle needs about a quarter of an hour per trajectory on 8 cores, so
-   these two run only when CHS_ORACLE_TRAJ=1 is set (measured: dwf3d's first
-   trajectory, 11 MD steps, dH -0.5748023034 against the program's
-   -0.5748022946, same decision; 928 s).  On the GPU the product's drivers
-   run every trajectory of every fixture (tests/test_gpu_ref_traj.py).
+   the oracle needs about 95 s for dwf3d's first trajectory (11 MD steps; dH
+   -0.5748023034 against the program's -0.5748022946, same decision) and 20 s
+   for the sibling's five, on 8 cores (with other OpenMP jobs on the same
+   cores it takes many times longer).  On the GPU the product's drivers run
+   every trajectory of every fixture (tests/test_gpu_ref_traj.py).
 
 dH tolerances: the sibling's CG force at the program's res = 1e-6 moves by
 3.8e-8 relative under a 1e-15 relative change of its source (finite-
@@ -66,11 +66,6 @@ def test_random_stream_consumption(case):
     assert int(g["seed0"]) * pow(MUL, N, MOD) % MOD == int(g["seed1"])
 
 
-slow = pytest.mark.skipif(os.environ.get("CHS_ORACLE_TRAJ") != "1",
-                          reason="about 15 minutes per trajectory on the CPU: set CHS_ORACLE_TRAJ=1")
-
-
-@slow
 def test_oracle_rhmc_first_trajectory_vs_reference_program():
     """orc_rhmc against program dwf3d's first trajectory (imass = 1, hpin 0.5,
     dt 0.125, the reference's thermalised rhmc/_con)."""
@@ -92,7 +87,6 @@ def test_oracle_rhmc_first_trajectory_vs_reference_program():
     assert abs(hp / KVOL - g["fort_11"][0, 2]) <= 1e-9 * abs(g["fort_11"][0, 2])
 
 
-@slow
 def test_oracle_hmc_run_vs_reference_program():
     """oracle.hmc_trajectory over the whole run of the sibling's program: every
     dH and dS, the actions of fort.11, the final chi and seed."""
