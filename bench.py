@@ -264,34 +264,45 @@ def sweep_bench(tol=1e-10, Ls=32):
                         "{0.45, 0.475, 0.5}, every storage form" % (tol, Ls), "rows": rows}
 
 
-def learned_bench(fm, lat, chi_d, b_d, tol, npairs=8, epochs=10, lr=2e-3):
-    """The learned preconditioner on the bench system: training pairs from the
-    plain FP32 solver, Adam training on the GPU, then flexible CG (FP32, FP64
-    reliable updates, plain-CG fallback) with and without the network."""
+def learned_bench(fm, lat, chi_d, b_d, tol, npairs=200, epochs=2, lr=1e-3):
+    """The learned preconditioner on the bench system: npairs (Gaussian source,
+    solution) pairs from the plain binary16 solver, Adam training on the GPU
+    (relative A-norm loss), then flexible CG (FP32, FP64 reliable updates,
+    plain-CG fallback) with and without the network, beside the plain
+    binary16 CG of the timed form."""
     from chiralheisenberg_amd import learn
     t0 = time.perf_counter()
-    src, sol, solver_s = learn.pairs_dev(lat, chi_d, npairs, rng_seed=3, tol=tol)
+    src, sol, solver_s = learn.pairs_dev(lat, chi_d, npairs, rng_seed=3, tol=tol, precision="fp16")
     t1 = time.perf_counter()
     pc = learn.Preconditioner(lat, hidden=16, seed=1)
-    losses = pc.train(src, sol, npairs, epochs=epochs, lr=lr)
+    losses = pc.train(src, sol, npairs, chi_d, epochs=epochs, lr=lr)
     lat.synchronize()
     t2 = time.perf_counter()
+    src.free()
+    sol.free()
     res = {}
     for use in (True, False):
         x = fm.spinor_dev(lat, None)
         x.upload(np.zeros(lat.shape, np.complex128))
         res[use] = learn.solve_fcg(lat, b_d, x, chi_d, tol, use_nn=use)
         x.free()
-    src.free()
-    sol.free()
+    x = fm.spinor_dev(lat, None)
+    x.upload(np.zeros(lat.shape, np.complex128))
+    h = fm.solve_mdagm(lat, b_d, x, chi_d, tol, "fp16")
+    x.free()
     nn, pl = res[True], res[False]
-    return {"pairs": npairs, "epochs": epochs, "pair_generation_s": t1 - t0, "training_s": t2 - t1,
+    return {"pairs": npairs, "pairs_why": "the north star's 2000 pairs take ~10 min of binary16 solves; the bench "
+                                          "keeps to a few minutes",
+            "epochs": epochs, "pair_generation_s": t1 - t0, "pair_solver_s": solver_s, "training_s": t2 - t1,
             "loss_first_last": [float(losses[0]), float(losses[-1])],
             "fcg_with_network": {"iterations": nn["iterations"], "ms": nn["seconds"] * 1e3,
                                  "fallbacks": nn["restarts"], "true_resid": nn["true_resid"]},
             "fcg_without_network": {"iterations": pl["iterations"], "ms": pl["seconds"] * 1e3,
                                     "true_resid": pl["true_resid"]},
-            "iteration_gain": pl["iterations"] / max(nn["iterations"], 1)}
+            "plain_fp16_cg": {"iterations": h["iterations"], "ms": h["seconds"] * 1e3, "true_resid": h["true_resid"]},
+            "iteration_gain_vs_fcg": pl["iterations"] / max(nn["iterations"], 1),
+            "iteration_gain_vs_plain_fp16": h["iterations"] / max(nn["iterations"], 1),
+            "time_gain_vs_plain_fp16": h["seconds"] / max(nn["seconds"], 1e-12)}
 
 
 def force_bench(fm, lat, chi, beta, res=1e-6, seed=5):
@@ -337,6 +348,9 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="only the solve forms at L in {16, 24}, Ls=32, beta in {0.45, 0.475, 0.5} (one JSON line)")
     ap.add_argument("--no-extras", action="store_true", help="skip the learned-preconditioner and force figures")
+    ap.add_argument("--learned-pairs", type=int, default=200,
+                    help="training pairs of the learned-preconditioner figure")
+    ap.add_argument("--learned-epochs", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box of this deployment has one MI355X, and the path "
@@ -437,6 +451,19 @@ def main():
     # spinor (2V) with the stencil's non-temporal stores
     ms_copy = fm.time_copy(lat, "fp64", nbuf=nb64, iters=200)
     copy_rate = 2 * V / (ms_copy * 1e-3)
+    # the streaming ceiling at >= 1 GB: an FP64 spinor of L = Lt = 56, Ls = 32
+    # (719 MB, so one copy moves 1.44 GB), with one element per thread and on
+    # persistent grids of 1024-8192 workgroups
+    big = fm.Lattice(56, 32)
+    big_bytes = 2 * big.kferm * 16
+    large_copy = {"bytes": big_bytes, "what": "plain FP64 copy of a 719 MB spinor (L=Lt=56, Ls=32), 2 buffer "
+                                              "pairs of random values, 40 launches"}
+    for grid in (0, 1024, 2048, 4096, 8192):
+        msb = fm.time_copy_grid(big, grid, nbuf=2, iters=40)
+        large_copy["full_grid" if grid == 0 else "persistent_%d" % grid] = {
+            "us": msb * 1e3, "GB_per_s": big_bytes / (msb * 1e-3) / 1e9,
+            "frac_of_peak": big_bytes / (msb * 1e-3) / HBM_PEAK}
+    big.close()
     # the same two over 4 sets, whose inputs stay in the Infinity Cache
     ms_warm = fm.time_dslash(lat, "fp64", False, nbuf=4, iters=200)
     ms_copy_warm = fm.time_copy(lat, "fp64", nbuf=4, iters=200)
@@ -480,6 +507,7 @@ def main():
                                       "us": ms_copy * 1e3, "what": "plain copy of one FP64 spinor (2V), "
                                       "non-temporal stores, same random-data buffers rotation"},
                      "frac_of_copy": achieved / copy_rate,
+                     "large_copy_ceiling": large_copy,
                      "infinity_cache_resident": {"nbuf": 4, "us": ms_warm * 1e3,
                                                  "GB_per_s": (2 * V + C) / (ms_warm * 1e-3) / 1e9,
                                                  "copy_us": ms_copy_warm * 1e3,
@@ -496,7 +524,8 @@ def main():
                                           "Cache" % nb_cg},
     }
     if not args.no_extras:
-        out["learned"] = learned_bench(fm, lat, chi_d, b_d, args.tol)
+        out["learned"] = learned_bench(fm, lat, chi_d, b_d, args.tol, npairs=args.learned_pairs,
+                                       epochs=args.learned_epochs)
         print("bench: learned %s" % json.dumps(out["learned"]), file=sys.stderr, flush=True)
         out["force"] = force_bench(fm, lat, chi, args.beta)
         print("bench: force %s" % json.dumps(out["force"]), file=sys.stderr, flush=True)

@@ -95,13 +95,14 @@ def _declare(L):
         "chs_time_dslash": ([v, i, i, i, i, d, i, ctypes.POINTER(ctypes.c_double)], i),
         "chs_time_cg_stencil": ([v, i, i, i, d, i, ctypes.POINTER(ctypes.c_double)], i),
         "chs_time_copy": ([v, i, i, i, ctypes.POINTER(ctypes.c_double)], i),
+        "chs_time_copy_grid": ([v, i, i, i, ctypes.POINTER(ctypes.c_double)], i),
         "chs_nn_init": ([v, i, ctypes.c_ulonglong], i),
         "chs_nn_num_weights": ([v], l),
         "chs_nn_set_weights": ([v, v, l], i),
         "chs_nn_get_weights": ([v, v, l], i),
         "chs_nn_apply_dev": ([v, v, v], i),
-        "chs_nn_loss_grad_dev": ([v, v, v, ctypes.POINTER(ctypes.c_double), v], i),
-        "chs_nn_train_dev": ([v, v, v, i, i, ctypes.c_float, v], i),
+        "chs_nn_loss_grad_dev": ([v, v, v, v, d, i, ctypes.POINTER(ctypes.c_double), v], i),
+        "chs_nn_train_dev": ([v, v, v, i, i, ctypes.c_float, v, d, i, v], i),
         "chs_solve_mdagm_fcg_dev": ([v, v, v, v, d, i, d, d, i, i, ctypes.POINTER(SolveStats)], i),
         "chs_batch_create": ([i], v),
         "chs_batch_destroy": ([v], None),

@@ -1416,7 +1416,7 @@ int chs_time_dslash(chs_ctx *c, int prec, int dagger, int nbuf, int iters, doubl
 // A plain copy of one spinor (2V bytes, non-temporal stores) over nbuf
 // rotating buffer pairs of random contents: the streaming rate the stencil's
 // HBM figure is set beside.
-int chs_time_copy(chs_ctx *c, int prec, int nbuf, int iters, double *avg_ms) {
+static int time_copy_impl(chs_ctx *c, int prec, int nbuf, int iters, long pgrid, double *avg_ms) {
     if (!prec_ok(prec) || nbuf < 1 || iters < 1) return fail(c, CHS_EINVAL, "bad arguments");
     if (c->kferm >= (1L << 32)) return fail(c, CHS_EINVAL, "spinor too large for the copy kernel");
     const size_t bytes = (size_t)c->kferm * prec_bytes(prec);
@@ -1430,7 +1430,10 @@ int chs_time_copy(chs_ctx *c, int prec, int nbuf, int iters, double *avg_ms) {
     const dim3 g(nblocks(n));
     auto launch = [&](int i) {
         void *o = outs[i % nbuf], *in = ins[i % nbuf];
-        if (prec == CHS_FP64)
+        if (prec == CHS_FP64 && pgrid > 0)
+            hipLaunchKernelGGL(k_copy_nt_grid<F64>, dim3((unsigned)pgrid), dim3(kBlock), 0, c->stream, n, (double2 *)o,
+                               (const double2 *)in);
+        else if (prec == CHS_FP64)
             hipLaunchKernelGGL(k_copy_nt<F64>, g, dim3(kBlock), 0, c->stream, n, (double2 *)o, (const double2 *)in);
         else if (prec == CHS_FP32)
             hipLaunchKernelGGL(k_copy_nt<F32>, g, dim3(kBlock), 0, c->stream, n, (float2 *)o, (const float2 *)in);
@@ -1451,6 +1454,14 @@ int chs_time_copy(chs_ctx *c, int prec, int nbuf, int iters, double *avg_ms) {
         hipFree(outs[i]);
     }
     return CHS_OK;
+}
+
+int chs_time_copy(chs_ctx *c, int prec, int nbuf, int iters, double *avg_ms) {
+    return time_copy_impl(c, prec, nbuf, iters, 0, avg_ms);
+}
+
+int chs_time_copy_grid(chs_ctx *c, int nbuf, int iters, int grid, double *avg_ms) {
+    return time_copy_impl(c, CHS_FP64, nbuf, iters, grid, avg_ms);
 }
 
 // The fused CG stencil of the reliable-update solve (M^dagger applied to M p

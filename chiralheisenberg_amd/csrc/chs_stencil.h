@@ -63,6 +63,16 @@ __global__ __launch_bounds__(kBlock) void k_copy_nt(long n, typename S::st *__re
     if (i < n) store_nt<S>(dst, (unsigned)i, S::ld(src, i));
 }
 
+// The same copy on a persistent grid: each workgroup strides over the array
+// (the streaming ceiling with a fixed number of resident workgroups).
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_copy_nt_grid(long n, typename S::st *__restrict__ dst,
+                                                         const typename S::st *__restrict__ src) {
+    const long stride = (long)gridDim.x * kBlock;
+    for (long i = logical_block() * kBlock + threadIdx.x; i < n; i += stride)
+        store_nt<S>(dst, (unsigned)i, S::ld(src, i));
+}
+
 // o += coef * (1 - e g_mu) (p0, p1, p2, p3).  MU = 0, 1, 2 (x, y, t) with
 // e = +1; MU = -1, -2, -3 the same with e = -1.
 template <class T, int MU>

@@ -390,6 +390,14 @@ def time_copy(lat, precision="fp64", nbuf=4, iters=100):
     return ms.value
 
 
+def time_copy_grid(lat, grid, nbuf=2, iters=40):
+    """Average ms per FP64 copy of one spinor by `grid` workgroups striding
+    over it (a persistent grid; grid <= 0: one element per thread)."""
+    ms = ctypes.c_double()
+    check(lat.h, lib().chs_time_copy_grid(lat.h, nbuf, iters, int(grid), ctypes.byref(ms)))
+    return ms.value
+
+
 def time_cg_stencil(lat, precision="fp32", nbuf=4, iters=100, hpin=0.0, imass=1):
     """Average ms per launch of the solve's fused CG stencil (HIP events)."""
     ms = ctypes.c_double()

@@ -1,11 +1,13 @@
 """Learned preconditioner for M^dagger M solves (host side, over libchs).
 
-The network lives on the GPU (chs_nn_* entry points): a 4-layer 3-D
-convolutional net over (x, y, t) on the Ls-stacked spinor, BF16 MFMA kernels
-for inference and training.  Training data are (source, solution) pairs of
-M^dagger M x = b produced by the plain solver; the trained network is the
-preconditioner of flexible CG, whose answer is judged by the FP64 true
-residual like every other solve (plain CG remains the fallback).
+The network lives on the GPU (chs_nn_* entry points): a linear, symmetric
+positive definite 3-D convolutional net on the Ls-stacked spinor,
+P(r) = c u + C1^T C2^T C2 C1 u (u = r / rms(r); C1 1x1x1, 16 Ls -> 16
+channels; C2 3x3x3 over x, y, t, 16 -> 16), BF16 MFMA kernels for inference
+and training.  Training data are (source, solution) pairs of M^dagger M x = b
+produced by the plain solver, the loss the A-norm of P's error; the trained
+network is the preconditioner of flexible CG, whose answer is judged by the
+FP64 true residual like every other solve (plain CG remains the fallback).
 """
 import ctypes
 
@@ -33,22 +35,26 @@ class Preconditioner:
         check(self.lat.h, lib().chs_nn_set_weights(self.lat.h, w.ctypes.data_as(ctypes.c_void_p), self.nweights))
 
     def apply(self, r_dev, z_dev):
-        """z = N(r) on FP32 native device spinors."""
+        """z = P(r) on FP32 native device spinors."""
         check(self.lat.h, lib().chs_nn_apply_dev(self.lat.h, r_dev.ptr, z_dev.ptr))
 
-    def loss_grad(self, src_dev, sol_dev):
+    def loss_grad(self, src_dev, sol_dev, chi_dev, hpin=0.0, imass=1):
+        """Relative A-norm loss of one pair and the gradient of all parameters
+        (A = M^dagger M of the operator given by chi_dev, hpin, imass)."""
         loss = ctypes.c_double()
         g = np.empty(self.nweights, dtype=np.float32)
-        check(self.lat.h, lib().chs_nn_loss_grad_dev(self.lat.h, src_dev.ptr, sol_dev.ptr, ctypes.byref(loss),
+        check(self.lat.h, lib().chs_nn_loss_grad_dev(self.lat.h, src_dev.ptr, sol_dev.ptr, chi_dev.ptr, float(hpin),
+                                                      int(imass), ctypes.byref(loss),
                                                       g.ctypes.data_as(ctypes.c_void_p)))
         return loss.value, g
 
-    def train(self, src_dev, sol_dev, npairs, epochs=1, lr=1e-3):
+    def train(self, src_dev, sol_dev, npairs, chi_dev, epochs=1, lr=1e-3, hpin=0.0, imass=1):
         """Adam over npairs pairs stored contiguously (FP32 native spinors);
         returns the mean loss of each epoch."""
         out = np.zeros(epochs)
         check(self.lat.h, lib().chs_nn_train_dev(self.lat.h, src_dev.ptr, sol_dev.ptr, int(npairs), int(epochs),
-                                                  float(lr), out.ctypes.data_as(ctypes.c_void_p)))
+                                                  float(lr), chi_dev.ptr, float(hpin), int(imass),
+                                                  out.ctypes.data_as(ctypes.c_void_p)))
         return out
 
 

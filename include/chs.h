@@ -268,30 +268,35 @@ int chs_solve_mdagm_dev(chs_ctx *ctx, const double *b, double *x,
                         chs_solve_stats *stats);
 
 /* ---- learned preconditioner (no counterpart in the reference) ------------
- * A 4-layer 3-D convolutional network (3x3x3 kernels over x, y, t, periodic
- * padding) on the Ls-stacked spinor: 16*Ls input/output channels
- * (s, spin-flavour plane, re/im), `hidden` channels between, tanh, no bias,
- * BF16 MFMA with FP32 accumulation, plus an identity path: with u = r / rms(r)
- * it returns u + net(u), an approximation of the direction of
- * (M^dagger M)^{-1} r, and is used inside flexible CG. */
+ * A linear, symmetric positive definite 3-D convolutional network on the
+ * Ls-stacked spinor (16*Ls channels: s, spin-flavour plane, re/im):
+ *   P(r) = c u + C1^T C2^T C2 C1 u,  u = r / rms(r),  c = exp(theta),
+ * C1 a 1x1x1 convolution 16*Ls -> 16 channels, C2 a 3x3x3 convolution over
+ * (x, y, t), periodic, 16 -> 16; BF16 operands on gfx950 MFMA
+ * (v_mfma_f32_16x16x32_bf16), FP32 accumulation.  P is SPD for any weights,
+ * so preconditioned CG keeps its conjugacy.  `hidden` must be 16. */
 int chs_nn_init(chs_ctx *ctx, int hidden, unsigned long long seed);
 long chs_nn_num_weights(chs_ctx *ctx);
-/* all weights, FP32, layers concatenated, each W[tap][cout][cin] */
+/* all parameters, FP32: W1[n][k] (16 x 16*Ls), W2[tap][n][k] (27 x 16 x 16,
+ * tap = (dx+1) + 3 (dy+1) + 9 (dt+1)), theta */
 int chs_nn_set_weights(chs_ctx *ctx, const float *w, long n);
 int chs_nn_get_weights(chs_ctx *ctx, float *w, long n);
-/* z = N(r): FP32 native-layout device spinors */
+/* z = P(r): FP32 native-layout device spinors */
 int chs_nn_apply_dev(chs_ctx *ctx, const float *r, float *z);
-/* loss (mean squared error of N(src) against sol / rms(sol)) and its gradient
- * (host array of chs_nn_num_weights floats, may be NULL) for one pair */
-int chs_nn_loss_grad_dev(chs_ctx *ctx, const float *src, const float *sol, double *loss, float *grad);
+/* loss and gradient (host array of chs_nn_num_weights floats, may be NULL)
+ * for one pair: with u = src / rms(src), w = P(u) - sol / rms(src), the
+ * relative A-norm <w, A w> / <u, sol / rms(src)>, A = M^dagger M of the
+ * operator given by chi, hpin, imass (device arrays) */
+int chs_nn_loss_grad_dev(chs_ctx *ctx, const float *src, const float *sol, const double *chi, double hpin, int imass,
+                         double *loss, float *grad);
 /* Adam training over npairs (source, solution) pairs stored contiguously as
  * FP32 native spinors; epoch_loss (epochs entries) receives the mean loss of
  * each epoch. */
 int chs_nn_train_dev(chs_ctx *ctx, const float *src, const float *sol, int npairs, int epochs, float lr,
-                     double *epoch_loss);
+                     const double *chi, double hpin, int imass, double *epoch_loss);
 /* Flexible CG (FP32 iteration, FP64 reliable updates every delta of
  * residual reduction, answer judged by the FP64 true relative residual) with
- * z = N(r) (use_nn = 1) or z = r (use_nn = 0, plain CG).  Plain-CG fallback:
+ * z = P(r) (use_nn = 1) or z = r (use_nn = 0, plain CG).  Plain-CG fallback:
  * when with the network |r|^2 has not fallen by a factor 8 within 64
  * iterations the solve continues from its current x as plain CG
  * (stats->restarts counts it). */
@@ -357,6 +362,12 @@ int chs_time_dslash(chs_ctx *ctx, int prec, int dagger, int nbuf, int iters,
  * the streaming rate the stencil's HBM figure is compared with.  A measuring
  * helper with no counterpart in the reference. */
 int chs_time_copy(chs_ctx *ctx, int prec, int nbuf, int iters, double *avg_ms);
+
+/* The same FP64 copy with `grid` workgroups of 256 threads striding over the
+ * spinor (grid <= 0: one element per thread, as chs_time_copy): the
+ * streaming ceiling of a persistent grid.  A measuring helper with no
+ * counterpart in the reference. */
+int chs_time_copy_grid(chs_ctx *ctx, int nbuf, int iters, int grid, double *avg_ms);
 
 /* Average duration in milliseconds of the fused CG stencil of
  * chs_solve_mdagm_dev (M^dagger applied to M p, fused with

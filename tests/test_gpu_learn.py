@@ -1,10 +1,12 @@
 """The learned preconditioner: GPU network (BF16 MFMA kernels) against the
 CPU oracle oracle/nn_oracle.py (same BF16 rounding points, FP64
-accumulation), flexible CG with and without the network, and training."""
+accumulation), its symmetry and positivity, flexible CG with and without the
+network, and training -- at a small lattice and at L = 24, Ls = 32."""
 import numpy as np
 import pytest
 
 import nn_oracle as nno
+import oracle as orc
 from conftest import phi_field, spinor
 
 pytestmark = pytest.mark.gpu
@@ -13,44 +15,80 @@ fm = pytest.importorskip("chiralheisenberg_amd.fermion")
 from chiralheisenberg_amd import learn  # noqa: E402
 
 
-def small():
-    lat = fm.Lattice(4, 2, 4)   # L=4, Lt=4, Ls=2: 32 channels, 64 sites
-    rs = np.random.RandomState(5)
+def net(L, Ls, Lt, seed=5, scale=0.3, theta=-1.0):
+    lat = fm.Lattice(L, Ls, Lt)
+    rs = np.random.RandomState(seed)
     pc = learn.Preconditioner(lat, hidden=16, seed=7)
-    w = (rs.randn(pc.nweights) * 0.1).astype(np.float32)
+    w = (rs.randn(pc.nweights) * scale).astype(np.float32)
+    w[-1] = theta
     pc.set_weights(w)
     return lat, pc, w, rs
 
 
-def test_network_forward_matches_oracle():
-    lat, pc, w, rs = small()
+def to_ref(lat, d):
+    return fm.to_host(lat, d)
+
+
+@pytest.mark.parametrize("L,Ls,Lt", [(4, 2, 4), (6, 4, 8), (5, 3, 3)])
+def test_network_forward_matches_oracle(L, Ls, Lt):
+    """P(r) on the GPU against the oracle, with bands of fewer than 4 rows
+    (L = 6, 5) and partial 16-site tiles (L = 5)."""
+    lat, pc, w, rs = net(L, Ls, Lt)
     src = spinor(rs, lat.shape).astype(np.complex64).astype(np.complex128)
     r = fm.spinor_dev(lat, src, "fp32")
     z = fm.spinor_dev(lat, None, "fp32")
     pc.apply(r, z)
-    got = fm.to_host(lat, z)                                # reference layout
-    Y, _ = nno.forward(src, nno.unpack_weights(w, lat.Ls), lat.Ls)
-    want_nat = nno.channels_to_spinor(Y, lat.Ls)            # native layout
-    want = np.moveaxis(want_nat, 2, 5)
+    got = to_ref(lat, z)
+    Z, _ = nno.forward(src, w, Ls)
+    want = nno.reference_from_native(nno.channels_to_spinor(Z, Ls))
     err = np.linalg.norm(got - want) / np.linalg.norm(want)
-    assert err < 5e-3, err
+    assert err < 2e-3, err
+    lat.close()
+
+
+def test_network_is_symmetric_positive_definite():
+    """<a, P b> = <P a, b> up to the BF16 rounding of the operands, and
+    <a, P a> >= c |a|^2 (P's input is a / rms(a), so both sides are taken
+    with that scaling)."""
+    lat, pc, w, rs = net(6, 4, 6)
+    a = spinor(rs, lat.shape).astype(np.complex64).astype(np.complex128)
+    b = spinor(rs, lat.shape).astype(np.complex64).astype(np.complex128)
+
+    def P(v):
+        d, z = fm.spinor_dev(lat, v, "fp32"), fm.spinor_dev(lat, None, "fp32")
+        pc.apply(d, z)
+        rms = np.sqrt(np.mean(np.abs(v) ** 2) / 2.0)
+        return to_ref(lat, z) * rms
+    Pa, Pb = P(a), P(b)
+    lhs, rhs = np.vdot(a, Pb).real, np.vdot(Pa, b).real
+    assert abs(lhs - rhs) <= 2e-2 * np.sqrt(np.vdot(a, Pa).real * np.vdot(b, Pb).real), (lhs, rhs)
+    assert np.vdot(a, Pa).real >= 0.99 * np.exp(w[-1]) * np.vdot(a, a).real
+    lat.close()
 
 
 def test_network_gradient_matches_oracle():
-    lat, pc, w, rs = small()
+    """Loss (relative A-norm of P's error, A = M^dagger M by the FP32 stencil)
+    and the gradients of every layer and of theta against the oracle (A in
+    FP64 by the CPU oracle's operator)."""
+    L, Ls, Lt = 4, 2, 6
+    lat, pc, w, rs = net(L, Ls, Lt, scale=0.2)
+    olat = orc.Lattice(L, Ls, Lt)
+    chi = phi_field(rs, lat.kvol)
+    chid = fm.DeviceArray.from_numpy(lat, chi)
     src = spinor(rs, lat.shape).astype(np.complex64).astype(np.complex128)
-    sol = spinor(rs, lat.shape).astype(np.complex64).astype(np.complex128)
-    loss, g = pc.loss_grad(fm.spinor_dev(lat, src, "fp32"), fm.spinor_dev(lat, sol, "fp32"))
-    loss_o, g_o = nno.loss_and_grad(src, sol, nno.unpack_weights(w, lat.Ls), lat.Ls)
-    assert abs(loss - loss_o) <= 1e-3 * loss_o, (loss, loss_o)
-    err = np.linalg.norm(g - g_o) / np.linalg.norm(g_o)
-    assert err < 2e-2, err
-    # per layer as well (a layer-local indexing error would hide in the total)
-    o = 0
-    for n in (27 * 16 * 32, 27 * 16 * 16, 27 * 16 * 16, 27 * 32 * 16):
-        e = np.linalg.norm(g[o:o + n] - g_o[o:o + n]) / np.linalg.norm(g_o[o:o + n])
-        assert e < 3e-2, (o, e)
-        o += n
+    sol = orc.congrad(olat, src, 1e-10, chi)[0].astype(np.complex64).astype(np.complex128)
+    loss, g = pc.loss_grad(fm.spinor_dev(lat, src, "fp32"), fm.spinor_dev(lat, sol, "fp32"), chid)
+
+    def A(v):
+        return orc.dslash(olat, orc.dslash(olat, v, chi, 0.0, 1, False), chi, 0.0, 1, True)
+    loss_o, g_o = nno.loss_and_grad(src, sol, w, Ls, A)
+    assert abs(loss - loss_o) <= 1e-3 * abs(loss_o), (loss, loss_o)
+    n1, n2 = 16 * 16 * Ls, 27 * 16 * 16
+    for name, sl in (("W1", slice(0, n1)), ("W2", slice(n1, n1 + n2))):
+        e = np.linalg.norm(g[sl] - g_o[sl]) / np.linalg.norm(g_o[sl])
+        assert e < 2e-2, (name, e)
+    assert abs(g[-1] - g_o[-1]) <= 1e-2 * abs(g_o[-1]), (g[-1], g_o[-1])
+    lat.close()
 
 
 def test_fcg_without_network_is_cg():
@@ -65,33 +103,65 @@ def test_fcg_without_network_is_cg():
     st_c = fm.solve_mdagm(lat, bd, x2, chid, 1e-10, "fp32")
     assert st_f["true_resid"] <= 1e-10 and st_c["true_resid"] <= 1e-10
     assert abs(st_f["iterations"] - st_c["iterations"]) <= 0.05 * st_c["iterations"] + 3, (st_f, st_c)
+    lat.close()
 
 
-def test_trained_network_gain():
-    """Training on (source, solution) pairs from the plain solver lowers the
-    loss; flexible CG with the trained network reaches the FP64 true residual
-    (through its plain-CG fallback when the network stalls), and the test
-    records what the network gains over the same solver without it.  On this
-    operator the measured gain is negative: the network's directions are
-    worse than plain CG's, which the fallback bounds."""
-    lat = fm.Lattice(8, 8)
-    rs = np.random.RandomState(13)
+def gain_run(L, Ls, npairs, epochs, lr, seed=13):
+    """Train on npairs (Gaussian source, solution) pairs of the plain solver,
+    then solve a fresh system with flexible CG with and without the network
+    and with the plain mixed-precision CG (binary16 form); returns the
+    statistics."""
+    lat = fm.Lattice(L, Ls)
+    rs = np.random.RandomState(seed)
     chi = phi_field(rs, lat.kvol)
     chid = fm.DeviceArray.from_numpy(lat, chi)
-    src, sol, _ = learn.pairs_dev(lat, chid, 16, rng_seed=3)
+    src, sol, _ = learn.pairs_dev(lat, chid, npairs, rng_seed=3, precision="fp16")
     pc = learn.Preconditioner(lat, hidden=16, seed=1)
-    losses = pc.train(src, sol, 16, epochs=30, lr=2e-3)
-    assert losses[-1] < 0.8 * losses[0], losses
+    losses = pc.train(src, sol, npairs, chid, epochs=epochs, lr=lr)
+    src.free()
+    sol.free()
     b = spinor(rs, lat.shape)
     bd = fm.spinor_dev(lat, b)
-    x1 = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
-    st_nn = learn.solve_fcg(lat, bd, x1, chid, 1e-10, use_nn=True)
-    x2 = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
-    st_id = learn.solve_fcg(lat, bd, x2, chid, 1e-10, use_nn=False)
-    print("loss %.4f -> %.4f; iterations with network %d (fallbacks %d), without %d; gain %.2fx" % (
-        losses[0], losses[-1], st_nn["iterations"], st_nn["restarts"], st_id["iterations"],
-        st_id["iterations"] / st_nn["iterations"]))
-    assert st_nn["true_resid"] <= 1e-10 and st_id["true_resid"] <= 1e-10
-    # the fallback bounds the loss (a restart discards the Krylov history, so
-    # the bound is loose)
-    assert st_nn["iterations"] <= 1.6 * st_id["iterations"] + 64 * st_nn["restarts"]
+    out = {"losses": losses}
+    for key, use in (("nn", True), ("fcg", False)):
+        x = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+        out[key] = learn.solve_fcg(lat, bd, x, chid, 1e-10, use_nn=use)
+        x.free()
+    x = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+    out["fp16"] = fm.solve_mdagm(lat, bd, x, chid, 1e-10, "fp16")
+    lat.close()
+    return out
+
+
+def report(tag, r):
+    nn, fc, h = r["nn"], r["fcg"], r["fp16"]
+    print("%s: loss %.4f -> %.4f; iterations / ms: FCG with network %d / %.1f (fallbacks %d), without %d / %.1f, "
+          "plain binary16 CG %d / %.1f; iteration gain %.3fx over FCG, %.3fx over the plain CG; time gain %.3fx" % (
+              tag, r["losses"][0], r["losses"][-1], nn["iterations"], nn["seconds"] * 1e3, nn["restarts"],
+              fc["iterations"], fc["seconds"] * 1e3, h["iterations"], h["seconds"] * 1e3,
+              fc["iterations"] / nn["iterations"], h["iterations"] / nn["iterations"],
+              h["seconds"] / nn["seconds"]), flush=True)
+
+
+def test_trained_network_gain_small():
+    """Training lowers the loss; flexible CG with the trained network reaches
+    the FP64 true residual, and the test records the gain over the same
+    solver without it (measured: below 1, see the network's design notes)."""
+    r = gain_run(8, 8, 16, 20, 2e-3)
+    report("L=8 Ls=8", r)
+    assert r["losses"][-1] < r["losses"][0], r["losses"]
+    for k in ("nn", "fcg", "fp16"):
+        assert r[k]["true_resid"] <= 1e-10, (k, r[k])
+    assert r["nn"]["iterations"] <= 2.0 * r["fcg"]["iterations"] + 64 * r["nn"]["restarts"]
+
+
+def test_trained_network_gain_l24_ls32():
+    """The north star's learned-preconditioner configuration (L = 24, Ls = 32,
+    beta = 0.475, mass 0.01) with 32 training pairs (the bench trains on
+    more): every solve reaches the FP64 true residual 1e-10; the gain over
+    flexible CG without the network and over the plain binary16 CG is
+    recorded, whatever its sign."""
+    r = gain_run(24, 32, 32, 4, 1e-3)
+    report("L=24 Ls=32", r)
+    for k in ("nn", "fcg", "fp16"):
+        assert r[k]["true_resid"] <= 1e-10, (k, r[k])
