@@ -161,6 +161,61 @@ def baseline_line(out, L, Ls, its, rel, threads, model, ncpu, cg_s, gpu_fp64):
 BATCH_MIXES = {"full": ((8, 8), (12, 16), (16, 32), (24, 32)), "small": ((8, 8), (12, 8), (12, 16), (16, 16))}
 
 
+def force_batch_bench(mix="full", npf=4, res=1e-6, seed=13, precision="fp64", beta=0.475):
+    """The pseudofermion force (both multi-shift stages, remez2g, the
+    reference's force tolerance 1e-6) of npf fields on every lattice of the
+    mix: each stage one batch (iflag = 0, then iflag = 1) versus the same two
+    solves per field one at a time on the device.  Returns forces/s of both
+    and whether the forces are identical."""
+    from chiralheisenberg_amd import fermion as fm, batch, remez
+    rz = remez.load("remez2g")
+    anum, aden, bnum, bden = rz[:4]
+    rs = np.random.RandomState(seed)
+    fields, lats = [], []
+    for L, Ls in BATCH_MIXES[mix]:
+        lat = fm.Lattice(L, Ls)
+        lats.append(lat)
+        chi = rs.randn(3, lat.kvol) / np.sqrt(beta)
+        chid = fm.DeviceArray.from_numpy(lat, chi)
+        for _ in range(npf):
+            phi = fm.spinor_dev(lat, (rs.randn(*lat.shape) + 1j * rs.randn(*lat.shape)) / np.sqrt(2))
+            fields.append((lat, phi, chid, chi))
+
+    def single():
+        Fs, its = [], []
+        for lat, phi, chid, chi in fields:
+            X = fm.spinor_dev(lat, None)
+            y = fm.spinor_dev(lat, None)
+            i1 = fm.qmrherm_prec_dev(lat, phi, res, bnum, bden, chid, X, precision)
+            F = fm.DeviceArray.from_numpy(lat, 2.0 * beta * chi)
+            i2 = fm.qmrherm_prec_dev(lat, X, res, anum, aden, chid, y, precision, iflag=1, F_dev=F)
+            Fs.append(F)
+            its.append((i1, i2))
+        for lat in lats:
+            lat.synchronize()
+        return Fs, its
+
+    single()   # warm-up (allocations)
+    t0 = time.perf_counter()
+    F1, its = single()
+    t_single = time.perf_counter() - t0
+    b = batch.Batch()
+    b.force(fields, beta, res, rz, precision)   # tunes and captures
+    t0 = time.perf_counter()
+    F2, _, (st1, st2) = b.force(fields, beta, res, rz, precision)
+    t_batch = time.perf_counter() - t0
+    same = all(np.array_equal(a.download(np.float64), c.download(np.float64)) for a, c in zip(F1, F2))
+    b.close()
+    return {"workload": "pseudofermion force x %d: remez2g (%d shifts, both stages) x %d fields x L/Ls %s, %s "
+                        "storage, res %g" % (len(fields), len(aden), npf,
+                                             ", ".join("%d/%d" % m for m in BATCH_MIXES[mix]), precision.upper(),
+                                             res),
+            "itercg": its, "one_at_a_time": {"seconds": t_single, "forces_per_s": len(fields) / t_single},
+            "batched": {"seconds": t_batch, "forces_per_s": len(fields) / t_batch,
+                        "tune": [st1["tune"], st2["tune"]]},
+            "speedup": t_single / t_batch, "identical_results": bool(same)}
+
+
 def batch_bench(mix="full", npf=4, res=1e-9, seed=11, precision="fp64"):
     """Batched multi-shift solves (all 12 remez2g shifts x npf pseudofermion
     fields x the lattices L/Ls of the mix, FP64) in one grouped,
@@ -361,8 +416,11 @@ def main():
         print(json.dumps(sweep_bench(args.tol, args.Ls)), flush=True)
         return
     if args.batch:
-        print(json.dumps({"%s_%s" % (m, p): batch_bench(m, precision=p) for m in BATCH_MIXES
-                          for p in ("fp64", "fp32")}), flush=True)
+        out = {"%s_%s" % (m, p): batch_bench(m, precision=p) for m in BATCH_MIXES for p in ("fp64", "fp32")}
+        for m in BATCH_MIXES:
+            for p in ("fp64", "fp32"):
+                out["force_%s_%s" % (m, p)] = force_batch_bench(m, precision=p)
+        print(json.dumps(out), flush=True)
         return
     from chiralheisenberg_amd import fermion as fm
     lat = fm.Lattice(args.L, args.Ls)

@@ -40,7 +40,7 @@ class MsSystem(ctypes.Structure):
     _fields_ = [("ctx", ctypes.c_void_p), ("phi", ctypes.c_void_p), ("chi", ctypes.c_void_p), ("x", ctypes.c_void_p),
                 ("hpin", ctypes.c_double), ("imass", ctypes.c_int), ("ndiag", ctypes.c_int),
                 ("anum", ctypes.c_void_p), ("aden", ctypes.c_void_p), ("res", ctypes.c_double),
-                ("itercg", ctypes.c_int)]
+                ("itercg", ctypes.c_int), ("iflag", ctypes.c_int), ("dsdchi", ctypes.c_void_p)]
 
 
 class BatchTune(ctypes.Structure):

@@ -3,8 +3,8 @@
 // lattices of different sizes), so that small lattices fill the GPU together
 // and a whole step is a handful of launches that a hipGraph replays.
 //
-// A system is one multi-shift solve (rhmc/cheisdwf.f:599-745, iflag = 0) with
-// its own geometry, source, auxiliary field, shifts and device work space;
+// A system is one multi-shift solve (rhmc/cheisdwf.f:599-745, iflag = 0 or 1)
+// with its own geometry, source, auxiliary field, shifts and device work space;
 // the spinors are stored in FP64 or FP32 (S) for the whole batch, the
 // Lanczos scalars, reductions and the rational combination are FP64.
 // Each grouped grid is the concatenation of the systems' own grids in system
@@ -36,6 +36,9 @@ struct MsSys {
     void *r, *q0, *q1, *v, *w, *pk;   // spinors in the batch's storage precision
     double2 *x;                   // the caller's output: a0 phi + sum_k anum[k+1] x_k (FP64)
     int pairs;                    // FP32 storage on an even L: the paired-site stencil
+    int iflag;                    // 1: every shift's x_k kept (on the wall slices) for the force
+    void *xk;                     // iflag = 1: the x_k, shift-major (storage precision)
+    XKeep xkeep;                  // iflag = 1: the slices of x_k that are stored
     const double *anum;           // device copy of the remez numerators
     LzState *lz;
     double *partial, *partial2;
@@ -120,6 +123,18 @@ __device__ __forceinline__ void ms_update(const MsSys &s, int parity, long lb) {
     const st *w = (const st *)s.w;
     st *r = (st *)s.r, *pk = (st *)s.pk;
     const int iter = s.lz->itercg + 1;
+    if (s.iflag) {   // the force's solve: x_k kept on the slices derivs reads (as k_lz_update)
+        if (s.ndiag == 25)
+            lz_update<S, 25, false, EPT>(s.n, w, qc, qp, r, pk, s.xk, s.anum, s.lz, s.ndiag, iter, s.partial2, lb,
+                                         s.xkeep);
+        else if (s.ndiag == 12)
+            lz_update<S, 12, false, EPT>(s.n, w, qc, qp, r, pk, s.xk, s.anum, s.lz, s.ndiag, iter, s.partial2, lb,
+                                         s.xkeep);
+        else
+            lz_update<S, 0, false, EPT>(s.n, w, qc, qp, r, pk, s.xk, s.anum, s.lz, s.ndiag, iter, s.partial2, lb,
+                                        s.xkeep);
+        return;
+    }
     if (s.ndiag == 25)
         lz_update<S, 25, true, EPT>(s.n, w, qc, qp, r, pk, s.x, s.anum, s.lz, s.ndiag, iter, s.partial2, lb);
     else if (s.ndiag == 12)

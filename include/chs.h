@@ -304,18 +304,20 @@ int chs_solve_mdagm_fcg_dev(chs_ctx *ctx, const double *b, double *x, const doub
                             double tol, double delta, int maxiter, int use_nn, chs_solve_stats *stats);
 
 /* ---- batched multi-shift solves (no counterpart in the reference) ----------
- * Many qmrherm solves (iflag = 0) at once -- pseudofermion fields, lattices
- * of different sizes -- iterated in lock step with one grouped launch per
+ * Many qmrherm solves at once -- pseudofermion fields, lattices of different
+ * sizes, the heatbath / Hamiltonian kind (iflag = 0) and the force kind
+ * (iflag = 1) mixed -- iterated in lock step with one grouped launch per
  * Lanczos stage, a chunk of steps replayed from a hipGraph.  Every system's
- * result and itercg equal those of chs_qmrherm_dev on the same inputs bit for
- * bit.  Systems may share a context (its geometry is all that is used); all
- * must be on the batch's device.  Device arrays in the native layout. */
+ * result (x, and for iflag = 1 the dsdchi contribution) and itercg equal
+ * those of chs_qmrherm_dev on the same inputs bit for bit.  Systems may share
+ * a context (its geometry is all that is used); all must be on the batch's
+ * device.  Device arrays in the native layout. */
 typedef struct chs_batch chs_batch;
 typedef struct {
     chs_ctx *ctx;           /* lattice geometry */
     const double *phi;      /* source (device, FP64) */
     const double *chi;      /* auxiliary field (device) */
-    double *x;              /* out: anum[0] phi + sum_k anum[k+1] x_k (device) */
+    double *x;              /* out: anum[0] phi + sum_k anum[k+1] x_k (iflag = 0), anum[0] phi (iflag = 1) */
     double hpin;
     int imass;
     int ndiag;
@@ -323,6 +325,8 @@ typedef struct {
     const double *aden;     /* host, ndiag */
     double res;             /* qmrherm's res: stop when max_k |rho_k| phimod < 2 sqrt(kferm) res */
     int itercg;             /* out */
+    int iflag;              /* 0, or 1: the force's solve (rhmc/cheisdwf.f:505-513, :733-740) */
+    double *dsdchi;         /* iflag = 1: (3, kvol) FP64 device array, the shifts' derivs accumulated */
 } chs_ms_system;
 /* launch parameters: ept = element blocks (of 256) per workgroup of the
  * shift update (1, 2 or 4), chunk = iterations per poll (even), graph =
