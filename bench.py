@@ -441,6 +441,11 @@ def main():
             nbc = cold_nbuf(lat.kferm * (16 if prec == "fp32" else 12))
             print("bench: cg stencil %s %.2f us" % (prec, fm.time_cg_stencil(lat, prec, nbuf=nbc, iters=20) * 1e3),
                   flush=True)
+        # the low-precision plain stencils (FP32 pairs, BF16, binary16)
+        for prec, bc in (("fp32", 8), ("bf16", 4), ("fp16", 4)):
+            nbl = cold_nbuf(lat.kferm * bc)
+            print("bench: dslash %s %.2f us" % (prec, fm.time_dslash(lat, prec, False, nbuf=nbl, iters=20) * 1e3),
+                  flush=True)
         return
     chi, b = make_inputs(lat, args.beta)
     chi_d = fm.DeviceArray.from_numpy(lat, chi)
