@@ -327,7 +327,10 @@ def learned_bench(fm, lat, chi_d, b_d, tol, npairs=200, epochs=2, lr=1e-3):
     binary16 CG of the timed form."""
     from chiralheisenberg_amd import learn
     t0 = time.perf_counter()
-    src, sol, solver_s = learn.pairs_dev(lat, chi_d, npairs, rng_seed=3, tol=tol, precision="fp16")
+    src, sol, solver_s = learn.pairs_dev(
+        lat, chi_d, npairs, rng_seed=3, tol=tol, precision="fp16",
+        progress=lambda i, s: print("bench: learned pairs %d / %d (solver %.1f s)" % (i, npairs, s), file=sys.stderr,
+                                    flush=True))
     t1 = time.perf_counter()
     pc = learn.Preconditioner(lat, hidden=16, seed=1)
     losses = pc.train(src, sol, npairs, chi_d, epochs=epochs, lr=lr)

@@ -67,11 +67,13 @@ def solve_fcg(lat, b_dev, x_dev, chi_dev, tol=1e-10, use_nn=True, hpin=0.0, imas
     return {f: getattr(st, f) for f, _ in _lib.SolveStats._fields_}
 
 
-def pairs_dev(lat, chi_dev, npairs, rng_seed=11, tol=1e-10, precision="fp32", kind="solve", hpin=0.0, imass=1):
+def pairs_dev(lat, chi_dev, npairs, rng_seed=11, tol=1e-10, precision="fp32", kind="solve", hpin=0.0, imass=1,
+              progress=None):
     """Generate npairs (source, solution) pairs of M^dagger M x = b.
     kind "solve": Gaussian b, x from the plain solver; kind "apply": Gaussian
     x, b = M^dagger M x (exact, no solve).  Returns (src, sol) FP32 native
-    device arrays (npairs spinors each) and the solver time in seconds."""
+    device arrays (npairs spinors each) and the solver time in seconds.
+    progress(i, seconds): called after every 100th pair, if given."""
     if kind not in ("solve", "apply"):
         raise ValueError("kind must be 'solve' or 'apply'")
     n = lat.kferm
@@ -99,5 +101,7 @@ def pairs_dev(lat, chi_dev, npairs, rng_seed=11, tol=1e-10, precision="fp32", ki
             check(lat.h, lib().chs_convert_dev(lat.h, ctypes.c_void_p(dst.ptr + i * n * 8), _lib.CHS_FP32, srcv.ptr,
                                                _lib.CHS_FP64, n))
         bd.free()
+        if progress is not None and (i + 1) % 100 == 0:
+            progress(i + 1, seconds)
     lat.synchronize()
     return src, sol, seconds
