@@ -90,7 +90,9 @@ def cpu_baseline(L, Ls, beta, gpu_fp64, full=False):
       - 100 dslash and 100 dslashd applications at L, Ls;
       - with full=True (bench.py --cpu-full; minutes on the CPU) also the
         config-2 multi-shift solve (L = 12, Ls = 24, remez4, res 1e-10) and
-        force (remez2g, res 1e-6).
+        force (remez2g, res 1e-6), and one config-1 trajectory (L = 8,
+        Ls = 8, dt = 0.05, iterl = 20, seeded) by the oracle's driver beside
+        the same trajectory by the GPU driver.
     Threads: OMP_NUM_THREADS when the environment sets it (the host's share
     for one GPU), else every CPU this process may use."""
     model, ncpu, allowed = host_cpu()
@@ -146,7 +148,35 @@ def cpu_baseline(L, Ls, beta, gpu_fp64, full=False):
           flush=True)
     out["config2"] = {"multishift_remez4_s": ms_s, "multishift_iterations": int(it_ms), "force_s": f_s,
                       "force_iterations": list(it_f) if hasattr(it_f, "__len__") else it_f}
+    out["config1_trajectory"] = trajectory_pair()
+    print("bench: config-1 trajectory %s" % json.dumps(out["config1_trajectory"]), file=sys.stderr, flush=True)
     return baseline_line(out, L, Ls, its, rel, threads, model, ncpu, cg_s, gpu_fp64)
+
+
+def trajectory_pair(L=8, Ls=8, seed=693764523745.0):
+    """One RHMC trajectory of config 1 (L = Ls = 8, beta 0.475, dt 0.05,
+    iterl 20, the reference's tolerances) from the same seeded phi and
+    generator, by the CPU oracle's driver and by the GPU driver: seconds,
+    MD steps, dH of each."""
+    import oracle as orc
+    from chiralheisenberg_amd import fermion as fm, remez, rhmc
+    par = dict(dt=0.05, beta=0.475, am=0.0, hpin=0.0, imass=1, iterl=20, itermax=1000, iprint=5, rescga=1e-9,
+               rescgg=1e-6)
+    sets = {k: remez.load(k) for k in ("remez2", "remez4", "remez2g")}
+    chi0 = np.random.RandomState(8).randn(3, L ** 3) / np.sqrt(par["beta"])
+    lat = fm.Lattice(L, Ls)
+    chi_g = chi0.copy()
+    t0 = time.perf_counter()
+    g = rhmc.trajectory(lat, chi_g, rhmc.RandomStream(seed), 1, par, sets)
+    t_gpu = time.perf_counter() - t0
+    lat.close()
+    olat = orc.Lattice(L, Ls)
+    chi_c = chi0.copy()
+    t0 = time.perf_counter()
+    c = orc.rhmc(olat, par, orc.Rng(seed), chi_c, 1, 1, sets)[0]
+    t_cpu = time.perf_counter() - t0
+    return {"L": L, "Ls": Ls, "md_steps": int(c["md_steps"]), "cpu_s": t_cpu, "gpu_s": t_gpu,
+            "dH_cpu": float(c["dH"]), "dH_gpu": float(g["dH"]), "same_decision": bool(c["accepted"] == g["accepted"])}
 
 
 def baseline_line(out, L, Ls, its, rel, threads, model, ncpu, cg_s, gpu_fp64):

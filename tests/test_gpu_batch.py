@@ -88,10 +88,11 @@ def make_force_systems(seed=21):
     of different sizes; the force arrays start from random values, to which
     the solves add."""
     rs = np.random.RandomState(seed)
-    specs = [((8, 8), R2G, 1, 2), ((6, 12), R2G, 1, 1), ((4, 4, 6), R4, 0, 1), ((10, 6), R2G, 1, 1),
-             ((5, 4), R2G, 0, 1), ((6, 3), R2G, 1, 1)]
+    # (dims, remez set, iflag, fields, imass, hpin): both wall couplings and the pinning field
+    specs = [((8, 8), R2G, 1, 2, 1, 0.0), ((6, 12), R2G, 1, 1, 3, 0.0), ((4, 4, 6), R4, 0, 1, 1, 0.5),
+             ((10, 6), R2G, 1, 1, 1, 0.5), ((5, 4), R2G, 0, 1, 3, 0.0), ((6, 3), R2G, 1, 1, 1, 0.0)]
     lats, systems, F0 = [], [], []
-    for dims, coef, iflag, npf in specs:
+    for dims, coef, iflag, npf, imass, hpin in specs:
         lat = fm.Lattice(*dims[:2], Lt=dims[2] if len(dims) > 2 else None)
         lats.append(lat)
         chi = fm.DeviceArray.from_numpy(lat, phi_field(rs, lat.kvol))
@@ -100,7 +101,8 @@ def make_force_systems(seed=21):
             x = fm.spinor_dev(lat, None)
             f0 = rs.randn(3, lat.kvol) if iflag else None
             F = fm.DeviceArray.from_numpy(lat, f0) if iflag else None
-            systems.append(batch.System(lat, phi, chi, x, coef[0], coef[1], res=1e-9, iflag=iflag, F=F))
+            systems.append(batch.System(lat, phi, chi, x, coef[0], coef[1], res=1e-9, hpin=hpin, imass=imass,
+                                        iflag=iflag, F=F))
             F0.append(f0)
     return lats, systems, F0
 
@@ -116,7 +118,8 @@ def test_batch_force_solves_equal_single(tune, precision):
     for s, f0 in zip(systems, F0):
         x = fm.spinor_dev(s.lat, None)
         F = fm.DeviceArray.from_numpy(s.lat, f0) if s.iflag else None
-        it = fm.qmrherm_prec_dev(s.lat, s.phi, s.res, s.anum, s.aden, s.chi, x, precision, iflag=s.iflag, F_dev=F)
+        it = fm.qmrherm_prec_dev(s.lat, s.phi, s.res, s.anum, s.aden, s.chi, x, precision, hpin=s.hpin,
+                                 imass=s.imass, iflag=s.iflag, F_dev=F)
         s.lat.synchronize()
         assert s.itercg == it and it > 10, (s.itercg, it)
         assert np.array_equal(fm.to_host(s.lat, s.x), fm.to_host(s.lat, x))
