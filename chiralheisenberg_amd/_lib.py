@@ -9,6 +9,7 @@ LIB_PATH = os.path.join(HERE, "libchs.so")
 
 CHS_OK, CHS_EINVAL, CHS_EHIP, CHS_ENOCONV, CHS_ENOMEM = 0, 1, 2, 3, 4
 CHS_FP64, CHS_FP32, CHS_BF16, CHS_FP16 = 0, 1, 2, 3
+CHS_LEARNED = 4   # chs_fermion's flexible CG with the learned preconditioner
 PRECISIONS = {"fp64": CHS_FP64, "fp32": CHS_FP32, "bf16": CHS_BF16, "fp16": CHS_FP16}
 BYTES_PER_COMPLEX = {CHS_FP64: 16, CHS_FP32: 8, CHS_BF16: 4, CHS_FP16: 4}
 MAX_SHIFTS = 32
@@ -103,6 +104,10 @@ def _declare(L):
         "chs_nn_apply_dev": ([v, v, v], i),
         "chs_nn_loss_grad_dev": ([v, v, v, v, d, i, ctypes.POINTER(ctypes.c_double), v], i),
         "chs_nn_train_dev": ([v, v, v, i, i, ctypes.c_float, v, d, i, v], i),
+        "chs_nn_collect": ([v, i], i),
+        "chs_nn_num_collected": ([v], i),
+        "chs_nn_pair_dev": ([v, i, ctypes.POINTER(v), ctypes.POINTER(v), ctypes.POINTER(v)], i),
+        "chs_nn_train_collected": ([v, i, ctypes.c_float, d, i, v], i),
         "chs_solve_mdagm_fcg_dev": ([v, v, v, v, d, i, d, d, i, i, ctypes.POINTER(SolveStats)], i),
         "chs_batch_create": ([i], v),
         "chs_batch_destroy": ([v], None),

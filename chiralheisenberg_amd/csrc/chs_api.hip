@@ -840,11 +840,16 @@ int chs_congrad_noalloc_dev(chs_ctx *c, const double *phi, double res, int *iter
 // correlators from the t-slice sums of x at s = Ls (cferm1, Dirac = source)
 // and s = 1 (cferm2, gamma_3 partner), with the antiperiodic sign for slices
 // behind the source.
+// appends (phi, x, chi) to the learned preconditioner's pair store while it
+// has room (chs_learn_host.inc)
+static int nn_store_pair(chs_ctx *c, const double2 *phi, const double2 *x, const double *chi);
+
 int chs_fermion(chs_ctx *c, const double *chi, double res, double hpin, int imass, int ittt0, int prec,
                 double *cferm1, double *cferm2, int *iterations) {
     if (ittt0 < 1 || ittt0 > c->Lt) return fail(c, CHS_EINVAL, "ittt0=%d outside 1..%d", ittt0, c->Lt);
-    if (prec != CHS_FP64 && prec != CHS_FP32 && prec != CHS_FP16)
-        return fail(c, CHS_EINVAL, "fermion precision must be FP64, FP32 or FP16");
+    if (prec != CHS_FP64 && prec != CHS_FP32 && prec != CHS_FP16 && prec != CHS_LEARNED)
+        return fail(c, CHS_EINVAL, "fermion solver form must be FP64, FP32, FP16 or LEARNED");
+    if (prec == CHS_LEARNED && !c->nn) return fail(c, CHS_EINVAL, "learned form needs chs_nn_init first");
     const int nsource = 5, Lt = c->Lt, Ls = c->Ls;
     const long n = c->kferm, L2 = (long)c->L * c->L;
     int rc = ensure_spinors(c, {B_PHI, B_X, B_Q0});
@@ -876,12 +881,18 @@ int chs_fermion(chs_ctx *c, const double *chi, double res, double hpin, int imas
                 HIPCHK(c, hipMemcpyAsync(&b2, c->dscal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
                 HIPCHK(c, hipStreamSynchronize(c->stream));
                 chs_solve_stats st;
-                rc = chs_solve_mdagm_dev(c, (const double *)phi, (double *)x, dchi, hpin, imass,
-                                         std::sqrt((double)n * res * res / b2), prec, 0.0, 0, &st);
+                const double tol = std::sqrt((double)n * res * res / b2);
+                if (prec == CHS_LEARNED)
+                    rc = chs_solve_mdagm_fcg_dev(c, (const double *)phi, (double *)x, dchi, hpin, imass, tol, 0.0,
+                                                 0, 1, &st);
+                else
+                    rc = chs_solve_mdagm_dev(c, (const double *)phi, (double *)x, dchi, hpin, imass, tol, prec, 0.0,
+                                             0, &st);
                 it = st.iterations;
             }
             if (rc == CHS_ENOCONV) status = rc;   // the reference carries on with the last iterate
             else if (rc) return rc;
+            if (prec != CHS_LEARNED && (rc = nn_store_pair(c, phi, x, dchi))) return rc;
             iter += it;
             SliceReq rq;
             rq.s[0] = Ls - 1, rq.plane[0] = 2 + ids;       // cferm1, flavour 1

@@ -191,13 +191,17 @@ def force(lat, Phi, res1, chi, beta, remez2g, am=0.0, hpin=0.0, imass=1):
 
 def fermion(lat, chi, res, ittt0, hpin=0.0, imass=1, precision="fp64"):
     """Wall-source correlators (rhmc/cheisdwf.f:897-1058) for the first source
-    slice ittt0 (1..Lt, the reference's int(Lt*rano)+1).  Returns (cferm1,
-    cferm2) complex arrays (2 flavours, Lt) and the total solver iterations
-    (aviter = iterations / 10)."""
+    slice ittt0 (1..Lt, the reference's int(Lt*rano)+1).  precision: the
+    solver form of the ten propagator solves, "fp64" (the reference CGNR),
+    "fp32", "fp16" or "learned" (flexible CG with the learned preconditioner
+    of learn.Preconditioner).  Returns (cferm1, cferm2) complex arrays
+    (2 flavours, Lt) and the total solver iterations (aviter = iterations /
+    10)."""
     c1 = np.zeros((2, lat.Lt), dtype=np.complex128)
     c2 = np.zeros_like(c1)
     it = ctypes.c_int(0)
-    check(lat.h, lib().chs_fermion(lat.h, _p(lat._field(chi)), res, hpin, imass, int(ittt0), PRECISIONS[precision],
+    form = _lib.CHS_LEARNED if precision == "learned" else PRECISIONS[precision]
+    check(lat.h, lib().chs_fermion(lat.h, _p(lat._field(chi)), res, hpin, imass, int(ittt0), form,
                                    _p(c1), _p(c2), ctypes.byref(it)))
     return c1, c2, it.value
 

@@ -8,6 +8,12 @@ and training.  Training data are (source, solution) pairs of M^dagger M x = b
 produced by the plain solver, the loss the A-norm of P's error; the trained
 network is the preconditioner of flexible CG, whose answer is judged by the
 FP64 true residual like every other solve (plain CG remains the fallback).
+
+Pairs come either from pairs_dev (Gaussian sources at one chi) or from the
+propagator solves themselves: Preconditioner.collect opens a device store
+that every plain-form fermion.fermion call fills (each pair with the chi it
+was solved with), train_collected trains on it, and fermion.fermion(...,
+precision="learned") then solves with the network.
 """
 import ctypes
 
@@ -56,6 +62,42 @@ class Preconditioner:
                                                   float(lr), chi_dev.ptr, float(hpin), int(imass),
                                                   out.ctypes.data_as(ctypes.c_void_p)))
         return out
+
+
+    def collect(self, max_pairs):
+        """Open a device store for max_pairs (source, solution) pairs of the
+        propagator solves of fermion.fermion in a plain form (emptying any
+        earlier one); 0 frees it."""
+        check(self.lat.h, lib().chs_nn_collect(self.lat.h, int(max_pairs)))
+
+    def num_collected(self):
+        return lib().chs_nn_num_collected(self.lat.h)
+
+    def pair(self, i):
+        """Stored pair i as host arrays: (source, solution) complex128 in the
+        reference layout (FP32 values) and chi (3, kvol)."""
+        ps, px, pc = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(self.lat.h, lib().chs_nn_pair_dev(self.lat.h, int(i), ctypes.byref(ps), ctypes.byref(px),
+                                                 ctypes.byref(pc)))
+        out = [fm.to_host(self.lat, _Fp32View(p.value)) for p in (ps, px)]
+        chi = np.empty((3, self.lat.kvol))
+        check(self.lat.h, lib().chs_download(self.lat.h, chi.ctypes.data_as(ctypes.c_void_p), pc.value, chi.nbytes))
+        return out[0], out[1], chi
+
+    def train_collected(self, epochs=1, lr=1e-3, hpin=0.0, imass=1):
+        """Adam over the stored pairs, each with its own chi; returns the mean
+        loss of each epoch."""
+        out = np.zeros(epochs)
+        check(self.lat.h, lib().chs_nn_train_collected(self.lat.h, int(epochs), float(lr), float(hpin), int(imass),
+                                                        out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+
+class _Fp32View:
+    """A non-owning FP32 native spinor at a device address (for fm.to_host)."""
+
+    def __init__(self, ptr):
+        self.ptr, self.prec = ptr, _lib.CHS_FP32
 
 
 def solve_fcg(lat, b_dev, x_dev, chi_dev, tol=1e-10, use_nn=True, hpin=0.0, imass=1, delta=0.1, maxiter=0):

@@ -141,7 +141,7 @@ def aux_observables(chi):
 
 
 def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, iread=1, iwrite=1, iseed=0,
-        rescgm=1e-9, force_precision="fp64", fermion_precision="fp64", sets=None):
+        rescgm=1e-9, force_precision="fp64", fermion_precision="fp64", sets=None, learned=None):
     """The main program dwf3d (rhmc/cheisdwf.f:85-460) with every fermion
     routine on the GPU.  workdir holds the inputs (midout unless par gives
     the parameters, con when iread = 1) and receives the reference's outputs:
@@ -149,7 +149,17 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
     accepted), fort.12/13/14 (cond, susc, condmod and susctot every iprint
     trajectories), fort.5001/5011/5002/5012 (fermion correlators), con (every
     icheck trajectories and at the end) and output (the averages).  Numbers
-    are written one record per line, space separated.  Returns the averages."""
+    are written one record per line, space separated.  Returns the averages.
+
+    learned = dict(pairs=N, epochs=E, lr=lr, seed=s) switches on the learned
+    preconditioner for the measurement's propagator solves: the
+    (source, solution) pairs of the first measurements' plain-form solves
+    (fermion_precision; ten per measurement) are stored on the device until
+    there are N, the network is trained on them (E epochs of Adam), and every
+    later measurement solves with flexible CG and the network.  The Markov
+    chain does not depend on it (the measurement draws the same random
+    number either way); the returned averages then hold "learned" (pairs,
+    per-epoch losses, the trajectory after which it was trained)."""
     from . import checkpoint
     p = dict(DEFAULTS)
     mid = os.path.join(workdir, "midout")
@@ -178,6 +188,12 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
     acc = dict(action=0.0, vel2=0.0, condmod=0.0, susc=0.0, cond=np.zeros(3), ancgm=0.0, yav=0.0, yyav=0.0,
                naccp=0, ipbp=0, itot=0, it_pf=0, it_h=0)
     gaction = paction = action = 0.0
+    pc, linfo = None, None
+    if learned:
+        from . import learn
+        pc = learn.Preconditioner(lat, seed=int(learned.get("seed", 1)))
+        pc.collect(int(learned["pairs"]))
+        linfo = dict(pairs=0, losses=[], trained_after=None)
     try:
         for isweep in range(1, iter2 + 1):
             t = trajectory(lat, chi, rng, isweep, p, sets, measure_draw=False)
@@ -206,7 +222,14 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
                 files["fort.13"].write(" ".join("%.17g" % v for v in susc) + "\n")
                 files["fort.14"].write("%.17g %.17g\n" % (condmod, susctot))
                 ittt0 = int(lat.Lt * rng.rano(0)) + 1
-                c1, c2, its = fm.fermion(lat, chi, rescgm, ittt0, p["hpin"], p["imass"], fermion_precision)
+                form = "learned" if linfo and linfo["trained_after"] is not None else fermion_precision
+                c1, c2, its = fm.fermion(lat, chi, rescgm, ittt0, p["hpin"], p["imass"], form)
+                if linfo and linfo["trained_after"] is None and pc.num_collected() >= int(learned["pairs"]):
+                    linfo["pairs"] = pc.num_collected()
+                    linfo["losses"] = pc.train_collected(int(learned.get("epochs", 1)), float(learned.get("lr", 1e-3)),
+                                                         p["hpin"], p["imass"]).tolist()
+                    linfo["trained_after"] = isweep
+                    pc.collect(0)
                 for it in range(lat.Lt):
                     for f, (u1, u2) in enumerate(((5001, 5011), (5002, 5012))):
                         files["fort.%d" % u1].write("%d %.17g %.17g\n" % (it, c1[f, it].real, c1[f, it].imag))
@@ -229,6 +252,8 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
     with open(os.path.join(workdir, "output"), "a") as f:
         for k, v in out.items():
             f.write("%s %s\n" % (k, v))
+    if linfo is not None:
+        out["learned"] = linfo
     if iwrite == 1:
         checkpoint.write_con(con, chi, rng.seed())
     lat.close()

@@ -47,6 +47,9 @@ enum {
 
 /* storage precision of a device spinor */
 enum { CHS_FP64 = 0, CHS_FP32 = 1, CHS_BF16 = 2, CHS_FP16 = 3 };
+/* solver form of chs_fermion's propagator solves beyond the storage
+ * precisions: flexible CG with the learned preconditioner (chs_nn_*) */
+enum { CHS_LEARNED = 4 };
 
 /* Upper bound on the number of rational shifts in one solve. */
 #define CHS_MAX_SHIFTS 32
@@ -164,7 +167,10 @@ int chs_hamilton(chs_ctx *ctx, const double *phi, double *h, double *hg,
  * of the routine, 1..Lt) passed in.  Five wall sources, Dirac 3 and 4,
  * congrad each (prec = CHS_FP64: the reference CGNR and stopping rule;
  * CHS_FP32 / CHS_FP16: the FP32 or scaled-binary16 reliable-update solve to
- * the same residual).
+ * the same residual; CHS_LEARNED: flexible CG with the learned
+ * preconditioner, chs_nn_init first, to the same residual).  While a pair
+ * store is open (chs_nn_collect), every solve of a CHS_FP64 / CHS_FP32 /
+ * CHS_FP16 call appends its (source, solution) pair to it.
  * cferm1 = fort.5001/5002, cferm2 = fort.5011/5012: complex (re, im)
  * [flavour][t], t fastest, 2*Lt complex each.  iterations = total solver
  * iterations (the reference's aviter times 10). */
@@ -294,6 +300,23 @@ int chs_nn_loss_grad_dev(chs_ctx *ctx, const float *src, const float *sol, const
  * each epoch. */
 int chs_nn_train_dev(chs_ctx *ctx, const float *src, const float *sol, int npairs, int epochs, float lr,
                      const double *chi, double hpin, int imass, double *epoch_loss);
+/* Training pairs from the propagator solves: chs_nn_collect opens a device
+ * store for max_pairs (source, solution) pairs -- FP32 native spinors and
+ * the chi each was solved with -- emptying any earlier one; max_pairs = 0
+ * frees it.  While it has room, every propagator solve of chs_fermion in a
+ * plain form (CHS_FP64 / CHS_FP32 / CHS_FP16) appends its pair (source
+ * M^dagger xi, solution x of M^dagger M x = M^dagger xi).  chs_nn_init
+ * (which recreates the network) frees the store. */
+int chs_nn_collect(chs_ctx *ctx, int max_pairs);
+/* number of pairs in the store (0 without a network or a store) */
+int chs_nn_num_collected(chs_ctx *ctx);
+/* device addresses of stored pair i: src and sol FP32 native spinors, chi
+ * 3*kvol doubles */
+int chs_nn_pair_dev(chs_ctx *ctx, int i, const float **src, const float **sol, const double **chi);
+/* chs_nn_train_dev over the stored pairs, each with the chi it was solved
+ * with; the same updates, in the same order, as chs_nn_train_dev on the
+ * same pairs when they share one chi */
+int chs_nn_train_collected(chs_ctx *ctx, int epochs, float lr, double hpin, int imass, double *epoch_loss);
 /* Flexible CG (FP32 iteration, FP64 reliable updates every delta of
  * residual reduction, answer judged by the FP64 true relative residual) with
  * z = P(r) (use_nn = 1) or z = r (use_nn = 0, plain CG).  Plain-CG fallback:
