@@ -404,15 +404,24 @@ def force_bench(fm, lat, chi, beta, res=1e-6, seed=5):
     out = {}
     F = {}
     for prec in ("fp64", "fp32"):
-        fm.set_force_precision(lat, prec)
-        fm.force(lat, phi, res, chi, beta, r2g)            # warm-up (allocations)
-        t0 = time.perf_counter()
-        F[prec], _, its = fm.force(lat, phi, res, chi, beta, r2g)
-        out[prec] = {"ms": (time.perf_counter() - t0) * 1e3, "iterations": list(its)}
+        for f in (0.0, 0.01):
+            key = prec if f == 0.0 else "%s_freeze%g" % (prec, f)
+            fm.set_force_precision(lat, prec)
+            lat.set_shift_freeze(f)
+            fm.force(lat, phi, res, chi, beta, r2g)            # warm-up (allocations)
+            t0 = time.perf_counter()
+            F[key], _, its = fm.force(lat, phi, res, chi, beta, r2g)
+            out[key] = {"ms": (time.perf_counter() - t0) * 1e3, "iterations": list(its)}
     fm.set_force_precision(lat, "fp64")
-    f64, f32 = F["fp64"] - 2 * beta * chi, F["fp32"] - 2 * beta * chi
-    out["fp32_rel_error"] = float(np.linalg.norm(f32 - f64) / np.linalg.norm(f64))
-    out["speedup"] = out["fp64"]["ms"] / out["fp32"]["ms"]
+    lat.set_shift_freeze(0.0)
+    f64 = F["fp64"] - 2 * beta * chi
+    for key in F:
+        if key != "fp64":
+            fk = F[key] - 2 * beta * chi
+            out[key]["rel_error"] = float(np.linalg.norm(fk - f64) / np.linalg.norm(f64))
+            out[key]["speedup"] = out["fp64"]["ms"] / out[key]["ms"]
+    out["fp32_rel_error"] = out["fp32"]["rel_error"]
+    out["speedup"] = out["fp32"]["speedup"]
     return out
 
 

@@ -75,6 +75,7 @@ def _declare(L):
         "chs_derivs": ([v, v, v, d, i, v], i),
         "chs_set_force_precision": ([v, i], i),
         "chs_set_lanczos_cap": ([v, i], i),
+        "chs_set_shift_freeze": ([v, d], i),
         "chs_mirror_register": ([v, v], i),
         "chs_mirror_touch": ([v, v], i),
         "chs_mirror_release": ([v, v], i),

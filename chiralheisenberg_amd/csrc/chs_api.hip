@@ -50,6 +50,7 @@ struct chs_ctx {
     int plane_stencil = 1;        // LDS-plane plain FP64 dslash when L^2 <= 1024 (CHS_PLANES=0: off)
     int force_prec = CHS_FP64;    // storage of the force's multi-shift solves (chs_set_force_precision)
     int lz_cap = 5000;            // multi-shift iteration cap (the reference's niterc)
+    double lz_freeze = 0.0;       // multi-shift shift freezing factor (chs_set_shift_freeze; 0 = off)
     void *scratch = nullptr;      // device address of the static scratch on this device
     std::string scratch_err;
     // Device mirrors of read-only host spinors, keyed by host address: the
@@ -561,6 +562,8 @@ int qmrherm_impl(chs_ctx *c, const double *phi, double res, int *itercg, double 
     LzState h;
     memset(&h, 0, sizeof h);
     h.ndiag = ndiag;
+    h.freeze = c->lz_freeze;
+    h.amask = ndiag == 32 ? ~0u : (1u << ndiag) - 1u;
     for (int k = 0; k < ndiag; k++) h.aden[k] = aden[k];
     HIPCHK(c, hipMemcpyAsync(c->lz, &h, sizeof h, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->dcoef, anum, sizeof(double) * (ndiag + 1), hipMemcpyHostToDevice, c->stream));
@@ -664,6 +667,12 @@ int chs_qmrherm_prec_dev(chs_ctx *c, const double *phi, double res, int *itercg,
     if (prec == CHS_FP64)
         return qmrherm_impl<F64>(c, phi, res, itercg, hpin, imass, anum, aden, ndiag, iflag, chi, x, dsdchi);
     return fail(c, CHS_EINVAL, "multi-shift precision must be FP64 or FP32");
+}
+
+int chs_set_shift_freeze(chs_ctx *c, double f) {
+    if (!(f >= 0.0 && f <= 1.0)) return fail(c, CHS_EINVAL, "shift freezing factor %g outside [0, 1]", f);
+    c->lz_freeze = f;
+    return CHS_OK;
 }
 
 int chs_set_lanczos_cap(chs_ctx *c, int niterc) {

@@ -296,6 +296,12 @@ struct LzState {
     double alphatild;
     double phimod, resid;
     int done, itercg, ndiag, pad;
+    // Shift freezing (0 = off, the reference's behaviour): once
+    // |phimod rho_k| < freeze * 2 resid, shift k's vectors are no longer
+    // updated (bit k of amask cleared); its scalars, and so the stopping
+    // test and itercg, run on unchanged.
+    double freeze;
+    unsigned amask, pad2;
     double aden[kMaxShift];
     double d[kMaxShift], rho[kMaxShift], amu[kMaxShift];
 };
@@ -362,6 +368,12 @@ __device__ __forceinline__ void lz_beta(LzState *st, const double *__restrict__ 
                 if (k == 0 || a > rhomax) rhomax = a;
             }
             if (rhomax < 2.0 * st->resid) st->done = 1;
+            if (st->freeze > 0.0) {
+                unsigned m = st->amask;
+                for (int k = 0; k < st->ndiag; k++)
+                    if (fabs(st->phimod * st->rho[k]) < st->freeze * 2.0 * st->resid) m &= ~(1u << k);
+                st->amask = m;
+            }
         }
     }
 }
@@ -418,7 +430,8 @@ __device__ __forceinline__ double lz_update_elem(long i, long n, const typename 
                                                  const typename S::st *__restrict__ qm1,
                                                  typename S::st *__restrict__ r, typename S::st *__restrict__ p,
                                                  void *__restrict__ xp, const LzState *__restrict__ st,
-                                                 const LzCoef &sc, int ndiag, int iter, XKeep xkeep = XKeep()) {
+                                                 const LzCoef &sc, int ndiag, int iter, unsigned amask,
+                                                 XKeep xkeep = XKeep()) {
     typedef cx<typename S::rt> C;
     const double at = st->alphatild, b0 = st->beta;
     const C qs = S::ld(q, i), ws = S::ld(w, i), qms = S::ld(qm1, i);
@@ -441,6 +454,7 @@ __device__ __forceinline__ double lz_update_elem(long i, long n, const typename 
         } else {
 #pragma unroll 4
             for (int k = 0; k < ns; k++) {
+                if (!((amask >> k) & 1u)) continue;
                 const C ps = S::ld(p, (long)k * n + i);
                 double2 pv;
                 pv.x = qv.x - sc.amu[k] * (double)ps.r;
@@ -465,6 +479,7 @@ __device__ __forceinline__ double lz_update_elem(long i, long n, const typename 
         } else if (kx) {
 #pragma unroll 4
             for (int k = 0; k < ns; k++) {
+                if (!((amask >> k) & 1u)) continue;
                 const C ps = S::ld(p, (long)k * n + i), xs = S::ld(x, (long)k * n + i);
                 double2 pv, xv;
                 pv.x = qv.x - sc.amu[k] * (double)ps.r;
@@ -477,6 +492,7 @@ __device__ __forceinline__ double lz_update_elem(long i, long n, const typename 
         } else {
 #pragma unroll 4
             for (int k = 0; k < ns; k++) {
+                if (!((amask >> k) & 1u)) continue;
                 const C ps = S::ld(p, (long)k * n + i);
                 double2 pv;
                 pv.x = qv.x - sc.amu[k] * (double)ps.r;
@@ -500,12 +516,13 @@ __device__ __forceinline__ void lz_update(long n, const typename S::st *__restri
                                           int iter, double *__restrict__ partial, long lb, XKeep xkeep = XKeep()) {
     __shared__ LzCoef sc;
     lz_coefs<CMB>(sc, st, ndiag, anum);
+    const unsigned amask = st->amask;   // uniform: the skips are scalar branches
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
         const long eb = lb * EPT + e;
         const long i = eb * kBlock + threadIdx.x;
         double part = 0.0;
-        if (i < n) part = lz_update_elem<S, NS, CMB>(i, n, w, q, qm1, r, p, x, st, sc, ndiag, iter, xkeep);
+        if (i < n) part = lz_update_elem<S, NS, CMB>(i, n, w, q, qm1, r, p, x, st, sc, ndiag, iter, amask, xkeep);
         const double t = block_sum(part);
         if (threadIdx.x == 0) partial[eb] = t;
     }

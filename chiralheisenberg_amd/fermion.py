@@ -106,6 +106,12 @@ class Lattice:
         """Iteration cap of the multi-shift solves (0: the reference's 5000)."""
         check(self.h, lib().chs_set_lanczos_cap(self.h, int(niterc)))
 
+    def set_shift_freeze(self, f):
+        """Shift freezing of the multi-shift solves: 0 (default) updates every
+        shift as the reference does; 0 < f <= 1 stops updating shift k's
+        vectors once |phimod rho_k| < f * 2 resid."""
+        check(self.h, lib().chs_set_shift_freeze(self.h, float(f)))
+
     def _spinor(self, a):
         a = np.ascontiguousarray(a, dtype=np.complex128)
         if a.size != self.kferm:

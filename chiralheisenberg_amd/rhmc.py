@@ -141,7 +141,7 @@ def aux_observables(chi):
 
 
 def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, iread=1, iwrite=1, iseed=0,
-        rescgm=1e-9, force_precision="fp64", fermion_precision="fp64", sets=None, learned=None):
+        rescgm=1e-9, force_precision="fp64", fermion_precision="fp64", sets=None, learned=None, shift_freeze=0.0):
     """The main program dwf3d (rhmc/cheisdwf.f:85-460) with every fermion
     routine on the GPU.  workdir holds the inputs (midout unless par gives
     the parameters, con when iread = 1) and receives the reference's outputs:
@@ -150,6 +150,10 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
     trajectories), fort.5001/5011/5002/5012 (fermion correlators), con (every
     icheck trajectories and at the end) and output (the averages).  Numbers
     are written one record per line, space separated.  Returns the averages.
+
+    shift_freeze = f > 0 freezes converged shifts in every multi-shift solve
+    (fermion.Lattice.set_shift_freeze; 0, the default, is the reference's
+    solve).
 
     learned = dict(pairs=N, epochs=E, lr=lr, seed=s) switches on the learned
     preconditioner for the measurement's propagator solves: the
@@ -171,6 +175,7 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
     icheck = icheck or iprint
     lat = fm.Lattice(L, Ls, Lt, am3=p.get("am3", 1.0))
     fm.set_force_precision(lat, force_precision)
+    lat.set_shift_freeze(shift_freeze)
     kv = lat.kvol
     con = os.path.join(workdir, "con")
     if iread == 1:

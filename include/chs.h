@@ -132,6 +132,20 @@ int chs_set_force_precision(chs_ctx *ctx, int prec);
  * earlier bound. */
 int chs_set_lanczos_cap(chs_ctx *ctx, int niterc);
 
+/* Shift freezing in every multi-shift solve of the context (no counterpart
+ * in the reference, which updates every shift's vectors until the slowest
+ * has converged, rhmc/cheisdwf.f:688-715): with 0 < f <= 1, shift k's p_k and
+ * x_k are no longer updated from the iteration after |phimod rho_k| falls
+ * below f * 2 resid -- the reference's stopping test applied to that shift
+ * alone, tightened by f.  The scalar recurrences of every shift continue, so
+ * the stopping test and itercg are unchanged; each frozen x_k lacks only
+ * updates smaller than f times what the reference's own stop leaves out.
+ * f = 0 (the default) updates every shift as the reference does.  For
+ * callers that want the force's solves faster (their largest shifts converge
+ * long before the smallest); the batched solve follows each system's
+ * context. */
+int chs_set_shift_freeze(chs_ctx *ctx, double f);
+
 /* ---- device mirrors of host spinors (boundary convention) ----------------
  * The host-array entry points upload each spinor argument once per call.
  * A read-only spinor argument that the caller passes again and again
