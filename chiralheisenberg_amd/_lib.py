@@ -76,6 +76,7 @@ def _declare(L):
         "chs_set_force_precision": ([v, i], i),
         "chs_set_lanczos_cap": ([v, i], i),
         "chs_set_shift_freeze": ([v, d], i),
+        "chs_qmrherm_shifts_dev": ([v, v, d, ctypes.POINTER(i), d, i, v, i, v, v], i),
         "chs_mirror_register": ([v, v], i),
         "chs_mirror_touch": ([v, v], i),
         "chs_mirror_release": ([v, v], i),

@@ -551,7 +551,8 @@ int qmrherm_impl(chs_ctx *c, const double *phi, double res, int *itercg, double 
     if (rc) return rc;
     if ((rc = ensure(c, c->shifts_p, (size_t)ndiag * n * sizeof(st)))) return rc;
     // iflag = 0 accumulates the rational combination directly into x; only
-    // the force (iflag = 1) needs every x_k
+    // the force (iflag = 1) needs every x_k; iflag = 2 (chs_qmrherm_shifts_dev)
+    // writes every x_k, whole, into the caller's x
     if (iflag == 1 && (rc = ensure(c, c->shifts_x, (size_t)ndiag * n * sizeof(st)))) return rc;
     st *r = buf<st>(c, B_R), *q0 = buf<st>(c, B_Q0), *q1 = buf<st>(c, B_Q1);
     st *v = buf<st>(c, B_V), *w = buf<st>(c, B_W);
@@ -569,8 +570,9 @@ int qmrherm_impl(chs_ctx *c, const double *phi, double res, int *itercg, double 
     HIPCHK(c, hipMemcpyAsync(c->dcoef, anum, sizeof(double) * (ndiag + 1), hipMemcpyHostToDevice, c->stream));
 
     // x = a0 Phi; r = Phi; q_{0} = 0; beta = phimod = |Phi|
-    hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, anum[0], (const double2 *)phi,
-                       0.0, (double2 *)x);
+    if (iflag != 2)
+        hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, anum[0], (const double2 *)phi,
+                           0.0, (double2 *)x);
     if (std::is_same<S, F64>::value)
         HIPCHK(c, hipMemcpyAsync(r, phi, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     else
@@ -656,17 +658,27 @@ extern "C" {
 int chs_qmrherm_dev(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass,
                     const double *anum, const double *aden, int ndiag, int iflag, const double *chi, double *x,
                     double *dsdchi) {
+    if (iflag != 0 && iflag != 1) return fail(c, CHS_EINVAL, "iflag=%d not 0 or 1", iflag);
     return qmrherm_impl<F64>(c, phi, res, itercg, hpin, imass, anum, aden, ndiag, iflag, chi, x, dsdchi);
 }
 
 int chs_qmrherm_prec_dev(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass,
                          const double *anum, const double *aden, int ndiag, int iflag, const double *chi, double *x,
                          double *dsdchi, int prec) {
+    if (iflag != 0 && iflag != 1) return fail(c, CHS_EINVAL, "iflag=%d not 0 or 1", iflag);
     if (prec == CHS_FP32)
         return qmrherm_impl<F32>(c, phi, res, itercg, hpin, imass, anum, aden, ndiag, iflag, chi, x, dsdchi);
     if (prec == CHS_FP64)
         return qmrherm_impl<F64>(c, phi, res, itercg, hpin, imass, anum, aden, ndiag, iflag, chi, x, dsdchi);
     return fail(c, CHS_EINVAL, "multi-shift precision must be FP64 or FP32");
+}
+
+int chs_qmrherm_shifts_dev(chs_ctx *c, const double *phi, double res, int *itercg, double hpin, int imass,
+                           const double *aden, int ndiag, const double *chi, double *xk) {
+    if (ndiag < 1 || ndiag > kMaxShift) return fail(c, CHS_EINVAL, "ndiag=%d outside 1..%d", ndiag, kMaxShift);
+    if (!xk) return fail(c, CHS_EINVAL, "xk is NULL");
+    const std::vector<double> anum(ndiag + 1, 0.0);   // no combination is formed
+    return qmrherm_impl<F64>(c, phi, res, itercg, hpin, imass, anum.data(), aden, ndiag, 2, chi, xk, nullptr);
 }
 
 int chs_set_shift_freeze(chs_ctx *c, double f) {

@@ -374,6 +374,28 @@ def qmrherm_prec_dev(lat, phi_dev, res, anum, aden, chi_dev, x_dev, precision="f
     return it.value
 
 
+def qmrherm_shifts(lat, Phi, res, aden, chi, hpin=0.0, imass=1):
+    """Every shifted solution x_k = (M^dagger M + aden[k])^{-1} Phi of the
+    multi-shift solve (the reference's stopping rule).  Returns (x_k complex128
+    (ndiag,) + lat.shape in the reference layout, itercg)."""
+    aden = np.ascontiguousarray(aden, dtype=np.float64)
+    nd, n = len(aden), lat.kferm
+    phi_dev = spinor_dev(lat, Phi)
+    chi_dev = DeviceArray.from_numpy(lat, lat._field(chi))
+    xk = DeviceArray(lat, nd * n * 16)
+    it = ctypes.c_int(0)
+    check(lat.h, lib().chs_qmrherm_shifts_dev(lat.h, phi_dev.ptr, res, ctypes.byref(it), hpin, imass, _p(aden), nd,
+                                              chi_dev.ptr, xk.ptr))
+    out = np.empty((nd,) + lat.shape, np.complex128)
+    ref = DeviceArray(lat, n * 16)
+    for k in range(nd):
+        check(lat.h, lib().chs_layout_dev(lat.h, ref.ptr, ctypes.c_void_p(xk.ptr + k * n * 16), CHS_FP64, 0))
+        out[k] = ref.download(np.complex128, lat.shape)
+    for d in (ref, xk, chi_dev, phi_dev):
+        d.free()
+    return out, it.value
+
+
 def qmrherm_dev(lat, phi_dev, res, anum, aden, chi_dev, x_dev, hpin=0.0, imass=1, iflag=0, F_dev=None):
     anum = np.ascontiguousarray(anum, dtype=np.float64)
     aden = np.ascontiguousarray(aden, dtype=np.float64)

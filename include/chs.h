@@ -83,7 +83,9 @@ int chs_dslashd(chs_ctx *ctx, double *phi, const double *r, const double *chi,
  * rhmc/cheisdwf.f:599.  The COMMON inputs become arguments: chi = /trial/
  * chit; x = /vector/ x (output: a0 Phi + sum a_k x_k for iflag=0, a0 Phi for
  * iflag=1); dsdchi = /sforce/ dSdchi (accumulated for iflag=1, may be NULL
- * for iflag=0).  anum has ndiag+1 entries, aden ndiag.  Returns CHS_ENOCONV,
+ * for iflag=0).  iflag must be 0 or 1 (CHS_EINVAL otherwise; the reference
+ * takes any nonzero value as 1, and its program passes only 0 and 1).  anum
+ * has ndiag+1 entries, aden ndiag.  Returns CHS_ENOCONV,
  * with x and dsdchi written as the reference leaves them, when the Lanczos
  * loop reaches its cap (niterc = 5000, rhmc/cheisdwf.f:603,639, where the
  * reference returns silently) or |r|^2 turns non-finite; chs_force and
@@ -131,6 +133,18 @@ int chs_set_force_precision(chs_ctx *ctx, int prec);
  * counterpart as a call: for tests of the cap and for callers that want an
  * earlier bound. */
 int chs_set_lanczos_cap(chs_ctx *ctx, int niterc);
+
+/* Every shifted solution of the multi-shift solve: x_k = (M^dagger M +
+ * aden[k])^{-1} Phi for k < ndiag, FP64 native device spinors, shift-major
+ * (x_k at xk + 2*kferm*k doubles), to the reference's stopping rule
+ * (max_k |phimod rho_k| < 2 sqrt(kferm) res).  No counterpart as a call: the
+ * reference keeps them in qmrherm's local x1(ndiag, kferm)
+ * (rhmc/cheisdwf.f:613, :672-687) and returns only their rational
+ * combination (iflag = 0) or force (iflag = 1); for callers that combine
+ * the shifts themselves (other coefficient sets over the same poles) and
+ * for checking each shift's true residual. */
+int chs_qmrherm_shifts_dev(chs_ctx *ctx, const double *phi, double res, int *itercg, double hpin, int imass,
+                           const double *aden, int ndiag, const double *chi, double *xk);
 
 /* Shift freezing in every multi-shift solve of the context (no counterpart
  * in the reference, which updates every shift's vectors until the slowest

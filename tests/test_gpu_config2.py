@@ -64,3 +64,39 @@ def test_hamilton_config2(setup):
     h, _, _ = fm.hamilton(lat, Phi, 1e-11, chi, pp, gen.BETA, remez.load("remez2"))
     for a, b in zip(h, ref["ham"]):
         assert abs(a - b) <= 1e-10 * abs(b), (h, ref["ham"])
+
+
+def test_qmrherm_shifts_config2(setup):
+    """Every shift's solution of the remez4 multi-shift solve (25 shifts) at
+    config 2, res 1e-10 with the reference's stopping rule: against the
+    oracle's x_k and each shift's FP64 true residual |Phi - (M^dagger M +
+    aden_k) x_k| / |Phi| (the GPU's FP64 operator, pinned to the oracle's
+    by the dslash test above)."""
+    import oracle as orc
+    lat, ref, (chi, R, Phi, pp, idx) = setup
+    a4, ad4 = remez.load("remez4")[:2]
+    xk, it = fm.qmrherm_shifts(lat, Phi, 1e-10, ad4, chi)
+    assert abs(it - int(ref["qmr_it"])) <= 5
+    # the combination of the shifts is the iflag = 0 solution
+    x = a4[0] * Phi + np.tensordot(a4[1:], xk, axes=1)
+    s = ref["qmr_s"]
+    assert np.abs(x.reshape(-1)[idx] - s).max() <= 1e-9 * np.abs(s).max()
+    worst = 0.0
+    pn = np.linalg.norm(Phi)
+    for k in range(len(ad4)):
+        Ax = fm.dslashd(lat, fm.dslash(lat, xk[k], chi, 0.0, 0.0, 1), chi, 0.0, 0.0, 1)
+        rk = np.linalg.norm(Phi - Ax - ad4[k] * xk[k]) / pn
+        worst = max(worst, rk)
+    print("remez4 at config 2: itercg %d, largest per-shift true residual %.2e" % (it, worst))
+    assert worst <= 1e-10
+    # the oracle's shift solutions on the smaller config-1 lattice
+    lat1, olat1 = fm.Lattice(8, 8), orc.Lattice(8, 8)
+    rs = np.random.RandomState(90)
+    phi1 = (rs.randn(*lat1.shape) + 1j * rs.randn(*lat1.shape)) / np.sqrt(2)
+    chi1 = rs.randn(3, lat1.kvol) / np.sqrt(gen.BETA)
+    xg, ig = fm.qmrherm_shifts(lat1, phi1, 1e-10, ad4, chi1)
+    _, ic, xc = orc.qmrherm(olat1, phi1, 1e-10, a4, ad4, chi1, want_shifts=True)
+    assert abs(ig - ic) <= 5
+    for k in range(len(ad4)):
+        assert np.abs(xg[k] - xc[k]).max() <= 1e-9 * np.abs(xc[k]).max(), k
+    lat1.close()

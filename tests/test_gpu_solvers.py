@@ -203,3 +203,13 @@ def test_solve_mdagm_true_residual(prec):
     rel = np.linalg.norm(r) / np.linalg.norm(b)
     assert rel <= 1.05e-10, (prec, rel, st)
     assert abs(st["true_resid"] - rel) <= 1e-3 * rel + 1e-16
+
+
+def test_qmrherm_rejects_iflag_other_than_0_or_1():
+    from chiralheisenberg_amd._lib import ChsError
+    lat, _ = lattices(4, 4)
+    rs = np.random.RandomState(93)
+    phi, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    for iflag in (2, -1):
+        with pytest.raises(ChsError):
+            fm.qmrherm(lat, phi, 1e-9, R4[0], R4[1], chi, 0.0, 0.0, 1, iflag, np.zeros((3, lat.kvol)))
