@@ -727,7 +727,8 @@ int chs_transfer_bytes(const chs_ctx *c, unsigned long long *h2d, unsigned long 
 }
 
 int chs_set_force_precision(chs_ctx *c, int prec) {
-    if (prec != CHS_FP64 && prec != CHS_FP32) return fail(c, CHS_EINVAL, "force precision must be FP64 or FP32");
+    if (prec != CHS_FP64 && prec != CHS_FP32 && prec != CHS_FP16)
+        return fail(c, CHS_EINVAL, "force precision must be FP64, FP32 or FP16");
     c->force_prec = prec;
     return CHS_OK;
 }
@@ -991,7 +992,9 @@ int chs_force(chs_ctx *c, const double *phi, double res1, double am, double hpin
     // X = r_b(M^dagger M) Phi with the remez2g b-set, then the force from the a-set
     // (rhmc/cheisdwf.f:500-513)
     // (solver storage per the context's force precision policy)
-    auto ms = c->force_prec == CHS_FP32 ? qmrherm_impl<F32> : qmrherm_impl<F64>;
+    // (CHS_FP16 runs the multi-shift solves in FP32: shifted systems have no
+    // reliable updates, and 16-bit storage does not converge there)
+    auto ms = c->force_prec != CHS_FP64 ? qmrherm_impl<F32> : qmrherm_impl<F64>;
     // a solve that stops at its cap (or on a non-finite residual) does not
     // stop the force: the reference carries on, and the status is returned
     // with the results written
@@ -1052,7 +1055,21 @@ int chs_hmc_force(chs_ctx *c, const double *phi, double *x0, double res, double 
     HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(dx1, dx0, 16 * n, hipMemcpyDeviceToDevice, c->stream));
     int it = 0;
-    rc = chs_congrad_dev(c, dphi, res, &it, 0.0, imass, dchi, dx1);
+    if (c->force_prec == CHS_FP64) {
+        rc = chs_congrad_dev(c, dphi, res, &it, 0.0, imass, dchi, dx1);
+    } else {
+        // the reliable-update CG (FP32, or binary16 search directions) from
+        // the same guess, to the congrad rule |r|^2 < kferm res^2 taken as an
+        // FP64 true relative residual
+        reduce_launch(c, n, (const double2 *)dphi, nullptr, 0, c->dscal);
+        double b2 = 0.0;
+        HIPCHK(c, hipMemcpyAsync(&b2, c->dscal, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const double tol = b2 > 0.0 ? std::sqrt((double)n * res * res / b2) : 1.0;   // Phi = 0: x = 0
+        chs_solve_stats st;
+        rc = chs_solve_mdagm_dev(c, dphi, dx1, dchi, 0.0, imass, tol, c->force_prec, 0.0, 0, &st);
+        it = st.iterations;
+    }
     if (rc && rc != CHS_ENOCONV) return rc;   // the sibling, too, carries on at its iteration cap
     // x0 = 2 X1 - x0 (mh/cheisdwf_hmc.f:434-438)
     hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, 2.0, (const double2 *)dx1, -1.0,

@@ -238,8 +238,11 @@ def hmc_hamilton(lat, Phi, x0, res, chi, pp, beta, imass=1, am=0.0):
 
 
 def set_force_precision(lat, precision):
-    """Storage precision ("fp64" or "fp32") of the multi-shift solves inside
-    force() for this lattice context; hamilton stays FP64."""
+    """Precision of the solves inside the force for this lattice context:
+    the multi-shift solves of force() in "fp64" or "fp32" storage ("fp16"
+    runs them in FP32); the CG of hmc_force() as the reference CGNR ("fp64")
+    or the reliable-update CG in "fp32" or "fp16" form.  hamilton stays
+    FP64."""
     check(lat.h, lib().chs_set_force_precision(lat.h, PRECISIONS[precision]))
 
 

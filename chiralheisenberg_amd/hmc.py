@@ -85,11 +85,14 @@ def read_midout(path):
     return dict(dt=dt, beta=beta, am3=am3, am=am, imass=imass, iterl=iterl, iter2=iter2)
 
 
-def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, iread=1, iwrite=1, iseed=0):
+def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, iread=1, iwrite=1, iseed=0,
+        force_precision="fp64"):
     """The sibling's main program: midout (unless par) and con in; control
     (dH, dS), fort.11 (isweep, gauge and momentum action per site), fort.12/13/14
     every iprint trajectories, con every icheck trajectories and at the end,
-    output (averages).  Returns the averages."""
+    output (averages).  force_precision: the force's CG, "fp64" (the
+    reference CGNR) or the reliable-update CG in "fp32" or "fp16" form;
+    hamilton stays FP64.  Returns the averages."""
     from . import checkpoint
     p = dict(DEFAULTS)
     mid = os.path.join(workdir, "midout")
@@ -98,6 +101,7 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, ire
     p.update(par or {})
     iter2 = int(iter2 if iter2 is not None else p.get("iter2", 1))
     lat = fm.Lattice(L, Ls, Lt, am3=p.get("am3", 1.0))
+    fm.set_force_precision(lat, force_precision)
     kv = lat.kvol
     con = os.path.join(workdir, "con")
     if iread == 1:

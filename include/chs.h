@@ -123,9 +123,14 @@ int chs_qmrherm_prec_dev(chs_ctx *ctx, const double *phi, double res, int *iterc
 
 /* Storage precision of the two multi-shift solves inside chs_force
  * (CHS_FP64, the default, or CHS_FP32: spinors in FP32, Lanczos scalars,
- * reductions and the derivs bilinears and sums in FP64).  No counterpart in
- * the reference (which computes in FP64 throughout); hamilton, and with it
- * the Metropolis test, stays FP64. */
+ * reductions and the derivs bilinears and sums in FP64; CHS_FP16 runs them
+ * in FP32 too, since shifted systems have no reliable updates and 16-bit
+ * storage does not converge there).  For the HMC sibling's chs_hmc_force
+ * the setting picks the CG: the reference CGNR (CHS_FP64) or the
+ * reliable-update CG of chs_solve_mdagm_dev in CHS_FP32 or CHS_FP16 form
+ * from the same guess, to the congrad rule taken as an FP64 true relative
+ * residual.  No counterpart in the reference (which computes in FP64
+ * throughout); hamilton, and with it the Metropolis test, stays FP64. */
 int chs_set_force_precision(chs_ctx *ctx, int prec);
 
 /* Iteration cap of every multi-shift solve of the context (the reference's

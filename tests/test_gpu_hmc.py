@@ -57,6 +57,53 @@ def test_hmc_force_vs_oracle_other_size():
     lat.close()
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_hmc_force_reduced_precision_vs_oracle(precision):
+    """The force's CG as the reliable-update CG (FP32 or binary16 search
+    directions, FP64 true residual at the congrad rule's threshold) from the
+    same guess: force and chronological guess against the oracle's FP64
+    congrad."""
+    lat, olat = fm.Lattice(6, 4, 8), orc.Lattice(6, 4, 8)
+    rs = np.random.RandomState(12)
+    phi, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    x0 = 0.1 * spinor(rs, lat.shape)
+    Fo, xo, ito = orc.hmc_force(olat, phi, x0, 1e-11, chi, 0.45, 3)
+    fm.set_force_precision(lat, precision)
+    xg = x0.copy()
+    F, it = fm.hmc_force(lat, phi, xg, 1e-11, chi, 0.45, 3)
+    ef = np.abs(F - Fo).max() / np.abs(Fo).max()
+    ex = np.abs(xg - xo).max() / np.abs(xo).max()
+    print("%s force CG: force error %.2e, guess error %.2e, iterations %d (oracle congrad %d)" % (
+        precision, ef, ex, it, ito))
+    assert ef <= 1e-8 and ex <= 1e-8
+    lat.close()
+
+
+def test_hmc_trajectories_fp16_force_match_oracle():
+    """Trajectories with the binary16-form force CG against the oracle's
+    FP64 loop: the same lengths and decisions, dH within 1e-6."""
+    L, Ls, ntraj = 4, 4, 6
+    par = dict(dt=0.06, beta=0.45, am=0.0, imass=1, iterl=3, itermax=1000, rescga=1e-11, rescgg=1e-10)
+    rs = np.random.RandomState(41)
+    chi0 = rs.randn(3, L ** 3) / np.sqrt(0.45)
+    seed = 693764523745.0
+    lat = fm.Lattice(L, Ls)
+    fm.set_force_precision(lat, "fp16")
+    chi_g = chi0.copy()
+    rng = hmc.RandomStream(seed)
+    gpu = [hmc.trajectory(lat, chi_g, rng, i + 1, par) for i in range(ntraj)]
+    olat = orc.Lattice(L, Ls)
+    chi_c = chi0.copy()
+    orng = orc.Rng(seed)
+    cpu = [orc.hmc_trajectory(olat, chi_c, orng, par) for _ in range(ntraj)]
+    ddh = max(abs(g["dH"] - c["dH"]) for g, c in zip(gpu, cpu))
+    print("fp16 force CG: accepted %s, max |dH - dH_oracle| %.2e" % ([g["accepted"] for g in gpu], ddh))
+    assert [g["md_steps"] for g in gpu] == [c["md_steps"] for c in cpu]
+    assert [g["accepted"] for g in gpu] == [c["accepted"] for c in cpu]
+    assert ddh <= 1e-6
+    lat.close()
+
+
 @pytest.mark.parametrize("imass", [1, 3])
 def test_hmc_trajectories_match_oracle(imass):
     L, Ls, ntraj = 4, 4, 6
