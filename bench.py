@@ -393,7 +393,7 @@ def learned_bench(fm, lat, chi_d, b_d, tol, npairs=200, epochs=2, lr=1e-3):
             "time_gain_vs_plain_fp16": h["seconds"] / max(nn["seconds"], 1e-12)}
 
 
-def force_bench(fm, lat, chi, beta, res=1e-6, seed=5):
+def force_bench(fm, lat, chi, beta, res=1e-6, seed=5, forms=("fp64", "fp32"), freezes=(0.0, 0.01)):
     """chs_force (remez2g, the reference's force tolerance 1e-6) with FP64 and
     with FP32-storage multi-shift solves: time and the relative difference of
     the fermion part."""
@@ -403,8 +403,8 @@ def force_bench(fm, lat, chi, beta, res=1e-6, seed=5):
     phi = (rs.randn(*lat.shape) + 1j * rs.randn(*lat.shape)) / np.sqrt(2)
     out = {}
     F = {}
-    for prec in ("fp64", "fp32"):
-        for f in (0.0, 0.01):
+    for prec in forms:
+        for f in freezes:
             key = prec if f == 0.0 else "%s_freeze%g" % (prec, f)
             fm.set_force_precision(lat, prec)
             lat.set_shift_freeze(f)
@@ -414,14 +414,16 @@ def force_bench(fm, lat, chi, beta, res=1e-6, seed=5):
             out[key] = {"ms": (time.perf_counter() - t0) * 1e3, "iterations": list(its)}
     fm.set_force_precision(lat, "fp64")
     lat.set_shift_freeze(0.0)
-    f64 = F["fp64"] - 2 * beta * chi
-    for key in F:
-        if key != "fp64":
-            fk = F[key] - 2 * beta * chi
-            out[key]["rel_error"] = float(np.linalg.norm(fk - f64) / np.linalg.norm(f64))
-            out[key]["speedup"] = out["fp64"]["ms"] / out[key]["ms"]
-    out["fp32_rel_error"] = out["fp32"]["rel_error"]
-    out["speedup"] = out["fp32"]["speedup"]
+    if "fp64" in F:
+        f64 = F["fp64"] - 2 * beta * chi
+        for key in F:
+            if key != "fp64":
+                fk = F[key] - 2 * beta * chi
+                out[key]["rel_error"] = float(np.linalg.norm(fk - f64) / np.linalg.norm(f64))
+                out[key]["speedup"] = out["fp64"]["ms"] / out[key]["ms"]
+    if "fp32" in out and "rel_error" in out["fp32"]:
+        out["fp32_rel_error"] = out["fp32"]["rel_error"]
+        out["speedup"] = out["fp32"]["speedup"]
     return out
 
 
@@ -445,6 +447,10 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="only the solve forms at L in {16, 24}, Ls=32, beta in {0.45, 0.475, 0.5} (one JSON line)")
     ap.add_argument("--no-extras", action="store_true", help="skip the learned-preconditioner and force figures")
+    ap.add_argument("--force-only", action="store_true",
+                    help="only the pseudofermion force figures (FP64 / FP32, with and without shift freezing)")
+    ap.add_argument("--force-forms", default="fp64,fp32", help="with --force-only: storage forms, comma-separated")
+    ap.add_argument("--force-freeze", default="0,0.01", help="with --force-only: shift-freezing factors")
     ap.add_argument("--learned-pairs", type=int, default=200,
                     help="training pairs of the learned-preconditioner figure")
     ap.add_argument("--learned-epochs", type=int, default=2)
@@ -466,6 +472,14 @@ def main():
         return
     from chiralheisenberg_amd import fermion as fm
     lat = fm.Lattice(args.L, args.Ls)
+    if args.force_only:
+        chi, _ = make_inputs(lat, args.beta)     # the default run's field
+        forms = tuple(args.force_forms.split(","))
+        freezes = tuple(float(f) for f in args.force_freeze.split(","))
+        print(json.dumps({"force": force_bench(fm, lat, chi, args.beta, forms=forms, freezes=freezes)}),
+              flush=True)
+        lat.close()
+        return
     if args.counters:
         # counter runs: plain copies of an FP64, an FP32 and a BF16 spinor
         # (k_convert: 16 / 8 / 4 B read and written per complex number) calibrate FETCH_SIZE /
