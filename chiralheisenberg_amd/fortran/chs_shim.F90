@@ -122,6 +122,16 @@ module chs_binding
       type(c_ptr), value :: c
       complex(c_double_complex) :: a(*)
     end function
+    integer(c_int) function c_set_force_precision(c, prec) bind(C, name="chs_set_force_precision")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+      integer(c_int), value :: prec
+    end function
+    integer(c_int) function c_set_shift_freeze(c, f) bind(C, name="chs_set_shift_freeze")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: c
+      real(c_double), value :: f
+    end function
   end interface
 
 contains
@@ -316,4 +326,23 @@ subroutine chs_touch(Phi)
   complex*16 Phi(kferm)
   if (.not. c_associated(ctx)) call chs_stop(c_null_ptr, 'chs_touch called before chs_mirror')
   call chs_check(c_mirror_touch(ctx, Phi), 'chs_touch')
+end subroutine
+
+! ---------------------------------------------------------------------------
+! Solver options of the shim's context (no counterpart in the reference):
+! iprec = 0 keeps the force's multi-shift solves in FP64 (the default), 1
+! stores their spinors in FP32 (chs_set_force_precision); freeze is the
+! shift-freezing factor f of every multi-shift solve, 0 (the default)
+! updating every shift as the reference does (chs_set_shift_freeze).
+! Call it after /para/ is set and before or between the other routines.
+
+subroutine chs_options(iprec, freeze)
+  use chs_binding
+  common /para/ bbb, am3, ibound
+  integer iprec
+  double precision freeze
+  call chs_ensure(real(am3, c_double), ibound)
+  if (iprec /= 0 .and. iprec /= 1) call chs_stop(c_null_ptr, 'chs_options: iprec must be 0 or 1')
+  call chs_check(c_set_force_precision(ctx, int(iprec, c_int)), 'chs_options')
+  call chs_check(c_set_shift_freeze(ctx, real(freeze, c_double)), 'chs_options')
 end subroutine

@@ -124,3 +124,25 @@ def test_shim_derivs_accumulates_into_sforce(shim, imass):
     shim.lib.derivs_(P(R), P(X2), byref(0.29), byref(imass, ctypes.c_int))
     G = orc.derivs(OLAT, F0.copy(), R, X2, 0.29, imass)
     assert np.abs(shim.F - G).max() <= 1e-13 * np.abs(G - F0).max()
+
+
+def test_shim_options_reduced_force(shim):
+    """chs_options (no counterpart in the reference) switches the shim's
+    context to FP32 force solves with shift freezing: the force stays within
+    the FP32 form's error of the oracle's (field-dependent: for this field
+    1.6e-4 without freezing, 2.0e-4 with it; bound 5e-4); chs_options(0, 0)
+    restores the reference's solves (the force to 1e-8 again)."""
+    rs = np.random.RandomState(6)
+    phi, chi = spinor(rs, SHAPE), phi_field(rs, KVOL)
+    shim.chit[:] = chi
+    Fc, _, _ = orc.force(OLAT, phi, 1e-9, chi, 0.475, remez.load("remez2g"))
+    shim.lib.chs_options_(byref(1, ctypes.c_int), byref(0.01))
+    shim.lib.force_(P(phi), byref(1e-9), byref(0.0), byref(0.0), byref(1, ctypes.c_int))
+    fr = Fc - 2 * 0.475 * chi
+    e = np.linalg.norm(shim.F - Fc) / np.linalg.norm(fr)
+    print("shim force with FP32 solves and freezing: error %.2e of the fermion part (largest element %.2e)" % (
+        e, np.abs(shim.F - Fc).max() / np.abs(fr).max()))
+    assert 0 < e <= 5e-4
+    shim.lib.chs_options_(byref(0, ctypes.c_int), byref(0.0))
+    shim.lib.force_(P(phi), byref(1e-9), byref(0.0), byref(0.0), byref(1, ctypes.c_int))
+    assert np.abs(shim.F - Fc).max() <= 1e-8 * np.abs(fr).max()
