@@ -101,13 +101,13 @@ __global__ __launch_bounds__(kBlock) void k_ms_dsl(MsGroup gr, int parity) {
     }
 }
 
-__global__ void k_ms_alpha(MsGroup gr) {
+__global__ __launch_bounds__(kBlock) void k_ms_alpha(MsGroup gr) {
     const MsSys &s = gr.sys[blockIdx.x];
     if (s.lz->done) return;
     lz_alpha(s.lz, s.partial, s.np1, s.lz->itercg + 1);
 }
 
-__global__ void k_ms_beta(MsGroup gr) {
+__global__ __launch_bounds__(kBlock) void k_ms_beta(MsGroup gr) {
     const MsSys &s = gr.sys[blockIdx.x];
     if (s.lz->done) return;
     lz_beta(s.lz, s.partial2, s.np2, s.lz->itercg + 1);

@@ -188,11 +188,15 @@ struct EpiArgs {
 // ---- deterministic FP64 reductions -----------------------------------------
 // One workgroup sums n partials in a fixed order: thread t adds part[t],
 // part[t + kBlock], ... in that order, then block_sum.  The sum is
-// latency-bound: up to 16 loads per thread are issued (predicated on the
-// bound) before the first addition, so n <= 16 * kBlock partials cost one
-// memory round trip.
+// latency-bound: up to kU loads per thread are issued (predicated on the
+// bound) before the first addition, so n <= kU * kBlock partials cost one
+// memory round trip.  kU changes the batching of the loads, not the order of
+// the additions, so every kU gives the same bits.  16 keeps the registers of
+// the stencils that inline it (the in-launch alpha) low; the one-workgroup
+// kernels over the Lanczos update's one partial per 256 elements (14,156 at
+// L=24, Ls=32) use 64.
+template <int kU = 16>
 __device__ __forceinline__ double sum_partials(const double *__restrict__ part, int n) {
-    constexpr int kU = 16;
     double v = 0.0;
     for (int i = threadIdx.x; i < n; i += kU * kBlock) {
         double t[kU];
@@ -346,7 +350,8 @@ __device__ __forceinline__ void lz_alpha(LzState *st, const double *__restrict__
     }
 }
 
-__global__ void k_lz_alpha(LzState *st, const double *__restrict__ partial, int np, int iter) {
+__global__ __launch_bounds__(kBlock) void k_lz_alpha(LzState *st, const double *__restrict__ partial, int np,
+                                                     int iter) {
     if (st->done) return;
     lz_alpha(st, partial, np, iter);
 }
@@ -355,7 +360,7 @@ __global__ void k_lz_alpha(LzState *st, const double *__restrict__ partial, int 
 // (rhmc/cheisdwf.f:697-715).  A non-finite |r|^2 halts the solve with
 // done = 2 (the host reports it; the reference has no such test).
 __device__ __forceinline__ void lz_beta(LzState *st, const double *__restrict__ partial, int np, int iter) {
-    const double t = sum_partials(partial, np);
+    const double t = sum_partials<64>(partial, np);
     if (threadIdx.x == 0) {
         st->beta = sqrt(t);
         st->itercg = iter;
@@ -378,7 +383,8 @@ __device__ __forceinline__ void lz_beta(LzState *st, const double *__restrict__ 
     }
 }
 
-__global__ void k_lz_beta(LzState *st, const double *__restrict__ partial, int np, int iter) {
+__global__ __launch_bounds__(kBlock) void k_lz_beta(LzState *st, const double *__restrict__ partial, int np,
+                                                    int iter) {
     if (st->done) return;
     lz_beta(st, partial, np, iter);
 }
