@@ -1275,9 +1275,9 @@ int rcg_solve(chs_ctx *c, const double *bvec, double *xvec, const double *chi, d
             }
         };
         // a solve whose true residual has not improved on its best in kStall
-        // reliable updates in a row is diverging (BF16 storage does, at L=16,
-        // Ls=32, beta=0.45): it stops with CHS_ENOCONV instead of running to
-        // maxiter.  Converging forms gain ~1/delta^2 in |r|^2 per update.
+        // reliable updates in a row, restarts included, is diverging: it stops
+        // with CHS_ENOCONV instead of running to maxiter.  Converging forms
+        // gain ~1/delta^2 in |r|^2 per update.
         constexpr int kStall = 8;
         double best_rn2 = rn2;
         int stall = 0;

@@ -295,10 +295,12 @@ typedef struct {
  * delta, delta <= 0 picks a default; in CHS_FP16 the search direction and its
  * image are stored in binary16 divided by the residual's rms, the residual
  * and the x increment in FP32).  x holds the initial guess on entry.
- * Returns CHS_ENOCONV after maxiter iterations (maxiter <= 0: 100000), on a
- * non-finite residual, or when the FP64 true residual has not improved in 8
- * reliable updates in a row (a diverging low-precision solve: BF16 storage
- * at L=16, Ls=32, beta=0.45 is one).  The CG step length alpha is formed
+ * A reliable update whose FP64 true residual is no better than the best so
+ * far restarts CG from p = r (stats->restarts; BF16 storage at L=16, Ls=32,
+ * beta=0.45 converges this way).  Returns CHS_ENOCONV after maxiter
+ * iterations (maxiter <= 0: 100000), on a non-finite residual, or when the
+ * true residual has not improved in 8 reliable updates in a row (a diverging
+ * solve).  The CG step length alpha is formed
  * inside the fused CG stencil launch, whose waves wait (bounded, about a
  * second) for it: CHS_EHIP if a wave gave up. */
 int chs_solve_mdagm_dev(chs_ctx *ctx, const double *b, double *x,
