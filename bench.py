@@ -89,10 +89,10 @@ def cpu_baseline(L, Ls, beta, gpu_fp64, full=False):
         relative residual of 1e-10 for a unit Gaussian source): `value`;
       - 100 dslash and 100 dslashd applications at L, Ls;
       - with full=True (bench.py --cpu-full; minutes on the CPU) also the
-        config-2 multi-shift solve (L = 12, Ls = 24, remez4, res 1e-10) and
-        force (remez2g, res 1e-6), and one config-1 trajectory (L = 8,
-        Ls = 8, dt = 0.05, iterl = 20, seeded) by the oracle's driver beside
-        the same trajectory by the GPU driver.
+        multi-shift solve at L = 12, Ls = 24 (remez4, res 1e-10) and the
+        force there (remez2g, res 1e-6), and one trajectory at L = 8, Ls = 8
+        (dt = 0.05, iterl = 20, seeded) by the oracle's driver beside the
+        same trajectory by the GPU driver.
     Threads: OMP_NUM_THREADS when the environment sets it (the host's share
     for one GPU), else every CPU this process may use."""
     model, ncpu, allowed = host_cpu()
@@ -132,7 +132,7 @@ def cpu_baseline(L, Ls, beta, gpu_fp64, full=False):
     out["cg_solve"] = {"s": cg_s, "iterations": int(its), "true_resid": rel}
     if not full:
         return baseline_line(out, L, Ls, its, rel, threads, model, ncpu, cg_s, gpu_fp64)
-    # config 2: L = 12, Ls = 24
+    # the multi-shift solve and force at L = 12, Ls = 24
     l2 = orc.Lattice(12, 24)
     rs = np.random.RandomState(2)
     chi2 = rs.randn(3, l2.kvol) / np.sqrt(beta)
@@ -144,17 +144,17 @@ def cpu_baseline(L, Ls, beta, gpu_fp64, full=False):
     t0 = time.perf_counter()
     _, _, it_f = orc.force(l2, phi2, 1e-6, chi2, beta, remez.load("remez2g"))
     f_s = time.perf_counter() - t0
-    print("bench: cpu config-2 multi-shift %.1f s (%d its), force %.1f s" % (ms_s, it_ms, f_s), file=sys.stderr,
+    print("bench: cpu L=12 Ls=24 multi-shift %.1f s (%d its), force %.1f s" % (ms_s, it_ms, f_s), file=sys.stderr,
           flush=True)
-    out["config2"] = {"multishift_remez4_s": ms_s, "multishift_iterations": int(it_ms), "force_s": f_s,
+    out["l12_ls24"] = {"multishift_remez4_s": ms_s, "multishift_iterations": int(it_ms), "force_s": f_s,
                       "force_iterations": list(it_f) if hasattr(it_f, "__len__") else it_f}
-    out["config1_trajectory"] = trajectory_pair()
-    print("bench: config-1 trajectory %s" % json.dumps(out["config1_trajectory"]), file=sys.stderr, flush=True)
+    out["trajectory_l8_ls8"] = trajectory_pair()
+    print("bench: L=8 Ls=8 trajectory %s" % json.dumps(out["trajectory_l8_ls8"]), file=sys.stderr, flush=True)
     return baseline_line(out, L, Ls, its, rel, threads, model, ncpu, cg_s, gpu_fp64)
 
 
 def trajectory_pair(L=8, Ls=8, seed=693764523745.0):
-    """One RHMC trajectory of config 1 (L = Ls = 8, beta 0.475, dt 0.05,
+    """One RHMC trajectory at L = Ls = 8 (beta 0.475, dt 0.05,
     iterl 20, the reference's tolerances) from the same seeded phi and
     generator, by the CPU oracle's driver and by the GPU driver: seconds,
     MD steps, dH of each."""
@@ -379,7 +379,7 @@ def learned_bench(fm, lat, chi_d, b_d, tol, npairs=200, epochs=2, lr=1e-3):
     h = fm.solve_mdagm(lat, b_d, x, chi_d, tol, "fp16")
     x.free()
     nn, pl = res[True], res[False]
-    return {"pairs": npairs, "pairs_why": "the north star's 2000 pairs take ~10 min of binary16 solves; the bench "
+    return {"pairs": npairs, "pairs_why": "2000 pairs take ~10 min of binary16 solves; the bench "
                                           "keeps to a few minutes",
             "epochs": epochs, "pair_generation_s": t1 - t0, "pair_solver_s": solver_s, "training_s": t2 - t1,
             "loss_first_last": [float(losses[0]), float(losses[-1])],
@@ -440,7 +440,7 @@ def main():
     ap.add_argument("--delta", type=float, default=0.0, help="reliable-update threshold (0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-full", action="store_true",
-                    help="CPU baseline also times the config-2 multi-shift solve and force (minutes)")
+                    help="CPU baseline also times the L=12 Ls=24 multi-shift solve and force, and an L=8 trajectory (minutes)")
     ap.add_argument("--counters", action="store_true",
                     help="only the roofline kernels and calibration copies (for the counter runs)")
     ap.add_argument("--batch", action="store_true", help="only the batched multi-shift workload (one JSON line)")

@@ -1,4 +1,4 @@
-"""Writes tests/golden/oracle_config2_l12_ls24.npz: the CPU oracle's results
+"""Writes tests/golden/oracle_ops_l12_ls24.npz: the CPU oracle's results
 for single operations at L=Lt=12, Ls=24, beta=0.475, imass=1 on seeded
 inputs (chi ~ N(0, 1/beta), complex Gaussian spinors, Gaussian momenta):
 dslash and dslashd (hpin 0.5; 4096 sampled entries and the 2-norm of each),
@@ -7,7 +7,7 @@ norm, iterations), the force with remez2g at res 1e-12, and hamilton with
 remez2 at res 1e-11.  The oracle needs a few minutes for these, so they are
 a fixture for the GPU test, which rebuilds the same inputs with inputs().
 
-    python tests/golden/make_oracle_config2_l12_ls24.py [threads]
+    python tests/golden/make_oracle_ops_l12_ls24.py [threads]
 """
 import os
 import sys
@@ -60,7 +60,7 @@ def main():
     h, _, ith = orc.hamilton(olat, Phi, 1e-11, chi, pp, BETA, remez.load("remez2"), 0.0, 1)
     out["ham"], out["ham_it"] = np.array(h), np.array(ith)
     print("hamilton done (%d iterations)" % ith, flush=True)
-    np.savez_compressed(os.path.join(HERE, "oracle_config2_l12_ls24.npz"), **out)
+    np.savez_compressed(os.path.join(HERE, "oracle_ops_l12_ls24.npz"), **out)
 
 
 if __name__ == "__main__":

@@ -1,4 +1,4 @@
-"""Config 4 at full size: batched multi-shift solves (qmrherm, rhmc/cheisdwf.f:
+"""The batched workload at full size: multi-shift solves (qmrherm, rhmc/cheisdwf.f:
 599-745, iflag = 0) of all 12 remez2g shifts x 4 pseudofermion fields over
 the lattice mix L/Ls = 8/8, 12/16, 16/32, 24/32 in one grouped,
 graph-replayed solve per batch, FP64 and FP32 storage.
@@ -48,14 +48,14 @@ def build(seed=4, res=1e-10):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_config4_batched_mix(precision):
+def test_batched_mix_l8_to_l24(precision):
     # FP32 storage serves the force's solves (the reference's rescgg = 1e-6):
     # its Lanczos vectors cannot carry a residual of 1e-10 (at L=24, Ls=32 it
     # stops at the 5000-iteration cap, reported as CHS_ENOCONV)
     res = 1e-10 if precision == "fp64" else 1e-6
     lats, systems, hosts = build(res=res)
     b = batch.Batch()
-    print("config 4 (%s): %d systems" % (precision, len(systems)), flush=True)
+    print("batched mix (%s): %d systems" % (precision, len(systems)), flush=True)
     st = b.qmrherm(systems, None, precision)
     print("  r_a batch: %.2f s, %d lock-step iterations, tune %s" % (st["seconds"], st["iterations"], st["tune"]),
           flush=True)

@@ -159,7 +159,7 @@ def test_trained_network_gain_small():
 
 
 def test_trained_network_gain_l24_ls32():
-    """The north star's learned-preconditioner configuration (L = 24, Ls = 32,
+    """The learned preconditioner's benchmark configuration (L = 24, Ls = 32,
     beta = 0.475, mass 0.01) with 32 training pairs (the bench trains on
     more): every solve reaches the FP64 true residual 1e-10; the gain over
     flexible CG without the network and over the plain binary16 CG is

@@ -1,6 +1,6 @@
-"""The workloads the north star names, at their full sizes, on the GPU.
+"""The benchmark workloads at their full sizes, on the GPU.
 
-Config 1 -- plain RHMC at L = 8, Ls = 8, beta = 0.475, M5 = 1, FP64: one
+Plain RHMC at L = 8, Ls = 8, beta = 0.475, M5 = 1, FP64: one
   trajectory of random length (dt = 0.05, iterl = 20: mean 20 steps) with
   every fermion routine on the GPU against the oracle's trajectory loop
   (orc_rhmc, itself pinned to the reference's own program by
@@ -8,7 +8,7 @@ Config 1 -- plain RHMC at L = 8, Ls = 8, beta = 0.475, M5 = 1, FP64: one
   0.5; and 100 applications of dslash and dslashd (50 rounds of the power
   iteration v <- M^dagger M v / |M^dagger M v| on a seeded Gaussian spinor)
   against the oracle's.
-Config 3 -- M^dagger M solves to an FP64 true relative residual of 1e-10 at
+Mixed precision -- M^dagger M solves to an FP64 true relative residual of 1e-10 at
   L = 16 and 24, Ls = 32, beta in {0.45, 0.475, 0.5}, in every reduced-
   precision form (FP32, scaled binary16 and BF16 storage with FP64 reliable
   updates): the true residual is recomputed here with the oracle's operator
@@ -28,7 +28,7 @@ from chiralheisenberg_amd import remez, rhmc  # noqa: E402
 
 
 @pytest.mark.parametrize("hpin", [0.0, 0.5])
-def test_config1_trajectory_l8_ls8(hpin):
+def test_trajectory_l8_ls8(hpin):
     L, Ls = 8, 8
     par = dict(dt=0.05, beta=0.475, am=0.0, hpin=hpin, imass=1, iterl=20, itermax=1000, iprint=5, rescga=1e-9,
                rescgg=1e-6)
@@ -44,7 +44,7 @@ def test_config1_trajectory_l8_ls8(hpin):
     chi_c = chi0.copy()
     orng = orc.Rng(seed)
     c = orc.rhmc(olat, par, orng, chi_c, 1, 1, sets)[0]
-    print("config 1 (hpin %g): %d MD steps, dH gpu %.10f oracle %.10f" % (hpin, g["md_steps"], g["dH"], c["dH"]),
+    print("L=8, Ls=8 trajectory (hpin %g): %d MD steps, dH gpu %.10f oracle %.10f" % (hpin, g["md_steps"], g["dH"], c["dH"]),
           flush=True)
     assert g["md_steps"] == c["md_steps"]
     assert g["accepted"] == c["accepted"]
@@ -55,7 +55,7 @@ def test_config1_trajectory_l8_ls8(hpin):
     assert rng.seed() == orng.get()
 
 
-def test_config1_hundred_dslash_dslashd():
+def test_hundred_dslash_dslashd_l8_ls8():
     L, Ls = 8, 8
     lat, olat = fm.Lattice(L, Ls), orc.Lattice(L, Ls)
     rs = np.random.RandomState(81)
@@ -73,7 +73,7 @@ def test_config1_hundred_dslash_dslashd():
 _FP64 = {}
 
 
-def _config3_system(L, beta):
+def _mixed_system(L, beta):
     """The bench's system: phi_a ~ N(0, 1/beta) and the unit complex Gaussian
     source drawn by the reference's generator (seed 693764523745), the inputs
     on which BF16 storage once diverged at L=16, beta=0.45."""
@@ -93,12 +93,12 @@ def _solve(lat, b, chi, prec):
 @pytest.mark.parametrize("L", [16, 24])
 @pytest.mark.parametrize("beta", [0.45, 0.475, 0.5])
 @pytest.mark.parametrize("prec", ["fp32", "fp16", "bf16"])
-def test_config3_mixed_precision_solves(L, beta, prec):
-    lat, b, chi = _config3_system(L, beta)
+def test_mixed_precision_solves_ls32(L, beta, prec):
+    lat, b, chi = _mixed_system(L, beta)
     if (L, beta) not in _FP64:
         _FP64[(L, beta)] = _solve(lat, b, chi, "fp64")
     x64, st64 = _FP64[(L, beta)]
-    print("config 3 L=%d beta=%g %s" % (L, beta, prec), flush=True)
+    print("mixed precision L=%d beta=%g %s" % (L, beta, prec), flush=True)
     x, st = _solve(lat, b, chi, prec)
     lat.close()
     olat = orc.Lattice(L, 32)
