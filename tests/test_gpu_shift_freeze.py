@@ -162,3 +162,32 @@ def test_driver_run_with_freezing(tmp_path):
     fa = [l.split()[0] for l in open(tmp_path / "f0" / "fort.11")]
     fb = [l.split()[0] for l in open(tmp_path / "f0.01" / "fort.11")]
     assert fa == fb
+
+
+def test_driver_run_fast_forms(tmp_path):
+    """rhmc.run with every opt-in fast form at once -- FP32 force solves,
+    shift freezing f = 0.01, binary16-form propagator solves -- against the
+    reference's solves: the same trajectory lengths and decisions, dH within
+    the FP32 force's error (1e-3 max(1, |dH|)), correlators within 1e-6."""
+    from chiralheisenberg_amd import checkpoint, rhmc
+    L, Ls, iter2 = 6, 8, 4
+    rs = np.random.RandomState(83)
+    chi0 = rs.randn(3, L ** 3) / np.sqrt(0.475)
+    runs = {}
+    for tag, kw in (("ref", {}), ("fast", dict(force_precision="fp32", shift_freeze=0.01, fermion_precision="fp16"))):
+        d = tmp_path / tag
+        d.mkdir()
+        open(d / "midout", "w").write("0.1 0.475 1.0 0.00 0.0 1 10 %d\ndt beta am3 am h0 imass iterl iter2\n" % iter2)
+        checkpoint.write_con(d / "con", chi0, 122110675517440.0)
+        rhmc.run(str(d), L, Ls, iprint=2, **kw)
+        runs[tag] = d
+    a = [l.split() for l in open(runs["ref"] / "control")]
+    b = [l.split() for l in open(runs["fast"] / "control")]
+    print("dH reference %s, fast forms %s" % ([x[0] for x in a], [x[0] for x in b]))
+    assert [x[2] for x in a] == [x[2] for x in b]
+    for x, y in zip(a, b):
+        assert abs(float(x[0]) - float(y[0])) <= 1e-3 * max(1.0, abs(float(x[0])))
+    for f in ("fort.5001", "fort.5011"):
+        p, q = np.loadtxt(runs["ref"] / f)[:, 1:], np.loadtxt(runs["fast"] / f)[:, 1:]
+        assert p.shape == q.shape
+        assert np.abs(p - q).max() <= 1e-6 * np.abs(p).max()
