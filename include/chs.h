@@ -346,7 +346,7 @@ int chs_nn_collect(chs_ctx *ctx, int max_pairs);
 /* number of pairs in the store (0 without a network or a store) */
 int chs_nn_num_collected(chs_ctx *ctx);
 /* device addresses of stored pair i: src and sol FP32 native spinors, chi
- * 3*kvol doubles */
+ * 3*kvol doubles; valid until the next chs_nn_collect or chs_nn_init */
 int chs_nn_pair_dev(chs_ctx *ctx, int i, const float **src, const float **sol, const double **chi);
 /* chs_nn_train_dev over the stored pairs, each with the chi it was solved
  * with; the same updates, in the same order, as chs_nn_train_dev on the
