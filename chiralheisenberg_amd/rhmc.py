@@ -165,6 +165,8 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
     number either way); the returned averages then hold "learned" (pairs,
     per-epoch losses, the trajectory after which it was trained)."""
     from . import checkpoint
+    if learned and int(learned.get("pairs", 0)) < 1:
+        raise ValueError("learned: 'pairs' must be >= 1")
     p = dict(DEFAULTS)
     mid = os.path.join(workdir, "midout")
     if par is None and os.path.exists(mid):
@@ -196,8 +198,6 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
     pc, linfo = None, None
     if learned:
         from . import learn
-        if int(learned.get("pairs", 0)) < 1:
-            raise ValueError("learned: 'pairs' must be >= 1")
         pc = learn.Preconditioner(lat, seed=int(learned.get("seed", 1)))
         pc.collect(int(learned["pairs"]))
         linfo = dict(pairs=0, losses=[], trained_after=None)
