@@ -65,3 +65,11 @@ def test_library_has_no_oracle_dependency():
     assert "orc_" not in out
     deps = os.popen("ldd %s" % _lib.LIB_PATH).read()
     assert "oracle" not in deps and "libref" not in deps
+
+
+def test_driver_options_checked_before_any_gpu_work(tmp_path):
+    """rhmc.run rejects a learned-preconditioner request without pairs before
+    it creates a lattice context (host logic; no GPU needed)."""
+    from chiralheisenberg_amd import rhmc
+    with pytest.raises(ValueError):
+        rhmc.run(str(tmp_path), 4, 4, learned=dict(pairs=0))
