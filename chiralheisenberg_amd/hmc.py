@@ -18,6 +18,7 @@ import os
 import numpy as np
 
 from . import fermion as fm
+from . import outfile
 from .rhmc import RandomStream, aux_observables, heatbath_source
 
 DEFAULTS = dict(dt=0.06, beta=0.45, am=0.0, imass=1, iterl=16, itermax=1000, rescga=1e-9, rescgg=1e-6)
@@ -90,7 +91,8 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, ire
     """The sibling's main program: midout (unless par) and con in; control
     (dH, dS), fort.11 (isweep, gauge and momentum action per site), fort.12/13/14
     every iprint trajectories, con every icheck trajectories and at the end,
-    output (averages).  force_precision: the force's CG, "fp64" (the
+    output (header, seeds and averages in the sibling's own formats).
+    force_precision: the force's CG, "fp64" (the
     reference CGNR) or the reliable-update CG in "fp32" or "fp16" form;
     hamilton stays FP64.  Returns the averages."""
     from . import checkpoint
@@ -112,6 +114,7 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, ire
         seed = 4587658465895.0      # mh/cheisdwf_hmc.f:82
     rng = RandomStream(seed)
     rng.rano(0)                      # y = rano(yran, idum) with idum = -1
+    outfile.header(os.path.join(workdir, "output"), seed, rng.seed(), L, lat.Lt, Ls, p, iter2, rhmc=False)
     files = {k: open(os.path.join(workdir, k), "a") for k in ("fort.11", "fort.12", "fort.13", "fort.14", "control")}
     acc = dict(action=0.0, vel2=0.0, condmod=0.0, susc=0.0, cond=np.zeros(3), yav=0.0, yyav=0.0, naccp=0, ipbp=0,
                itot=0, it_f=0, it_h=0)
@@ -156,9 +159,16 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, ire
                it_force=acc["it_f"] / max(acc["itot"], 1), it_acceptance=acc["it_h"] / (2 * n),
                condmod=acc["condmod"] / ip, susc=acc["susc"] / ip, cond=(acc["cond"] / ip).tolist(),
                vel2=acc["vel2"] / n, action=acc["action"] / n)
-    with open(os.path.join(workdir, "output"), "a") as f:
-        for k, v in out.items():
-            f.write("%s %s\n" % (k, v))
+    # measurement and psibarpsi: the sibling never sets ancgm and pbp (its
+    # estimator call is commented out, mh/cheisdwf_hmc.f:321); its build
+    # prints NaN for their averages, and so does this file
+    nan = float("nan")
+    outfile.averages(os.path.join(workdir, "output"),
+                     dict(naccp=acc["naccp"], itot=acc["itot"], yav=acc["yav"], yyav=acc["yyav"],
+                          guidance=acc["it_f"], acceptance=acc["it_h"], measurement=nan, ipbp=acc["ipbp"],
+                          condmod=acc["condmod"], susc=acc["susc"], cond=acc["cond"], pbp=nan, vel2=acc["vel2"],
+                          action=acc["action"]),
+                     n, p["dt"], seed_end=rng.seed() if iwrite == 1 else None, rhmc=False)
     if iwrite == 1:
         checkpoint.write_con(con, chi, rng.seed())
     lat.close()

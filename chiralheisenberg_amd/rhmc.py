@@ -20,6 +20,7 @@ import os
 import numpy as np
 
 from . import fermion as fm
+from . import outfile
 from . import remez as rz
 from ._lib import Rng, lib
 
@@ -63,20 +64,29 @@ def heatbath_source(lat, rng):
     return R
 
 
-def leapfrog(lat, Phi, chit, pp, par, remez2g, stop):
+def leapfrog(lat, Phi, chit, pp, par, remez2g, stop, its=None):
     """The molecular-dynamics evolution of rhmc/cheisdwf.f:261-315, in place
     on chit and pp: a half momentum step, then steps of chit += dt pp and the
     force; after each step's force stop(step) decides the end, and the last
     momentum step is a half step (a full one when itermax ends the loop).
     The reference's stop is one rano(yrun) < 1/iterl draw per step.  Returns
-    the number of steps."""
+    the number of steps; its (a list of two, if given) receives the force's
+    multi-shift iterations, guidance and force solves (the reference's ancg
+    and ancgf, rhmc/cheisdwf.f:507,515)."""
     dt = par["dt"]
     args = (par["rescgg"], chit, par["beta"], remez2g, par["am"], par["hpin"], par["imass"])
-    F, _, _ = fm.force(lat, Phi, *args)
+
+    def force():
+        F, _, (i1, i2) = fm.force(lat, Phi, *args)
+        if its is not None:
+            its[0] += i1
+            its[1] += i2
+        return F
+    F = force()
     pp -= 0.5 * dt * F
     for it in range(1, par["itermax"] + 1):
         chit += dt * pp
-        F, _, _ = fm.force(lat, Phi, *args)
+        F = force()
         if stop(it):
             pp -= 0.5 * dt * F
             return it
@@ -107,7 +117,8 @@ def trajectory(lat, chi, rng, isweep, par=None, sets=None, measure_draw=True):
     (H0, hg0, hp0, S0), _, it_h0 = fm.hamilton(lat, Phi, p["rescga"], chit, pp, p["beta"], sets["remez2"],
                                                p["am"], p["hpin"], p["imass"], isweep)
     proby = 1.0 / p["iterl"]
-    steps = leapfrog(lat, Phi, chit, pp, p, sets["remez2g"], lambda it: rng.rano(1) < proby)
+    its = [0, 0]
+    steps = leapfrog(lat, Phi, chit, pp, p, sets["remez2g"], lambda it: rng.rano(1) < proby, its)
     (H1, hg1, hp1, S1), _, it_h1 = fm.hamilton(lat, Phi, p["rescga"], chit, pp, p["beta"], sets["remez2"],
                                                p["am"], p["hpin"], p["imass"], isweep)
     dH = H0 - H1
@@ -118,7 +129,7 @@ def trajectory(lat, chi, rng, isweep, par=None, sets=None, measure_draw=True):
         rng.rano(0)
     return dict(H0=H0, H1=H1, S0=S0, S1=S1, dH=dH, dS=S0 - S1, accepted=int(accepted), md_steps=steps,
                 hg0=hg0, hp0=hp0, hg1=hg1, hp1=hp1, vel2=float(np.sum(pp * pp) / kv), it_pf=it_pf,
-                it_h0=it_h0, it_h1=it_h1)
+                it_h0=it_h0, it_h1=it_h1, it_guidance=its[0], it_force=its[1])
 
 
 def read_midout(path):
@@ -148,8 +159,10 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
     fort.11 (isweep, gauge and momentum action per site), control (dH, dS,
     accepted), fort.12/13/14 (cond, susc, condmod and susctot every iprint
     trajectories), fort.5001/5011/5002/5012 (fermion correlators), con (every
-    icheck trajectories and at the end) and output (the averages).  Numbers
-    are written one record per line, space separated.  Returns the averages.
+    icheck trajectories and at the end) and output (header, seeds and
+    averages in the program's own formats 9001/9004/9022).  The other files'
+    numbers are written one record per line, space separated.  Returns the
+    averages.
 
     shift_freeze = f > 0 freezes converged shifts in every multi-shift solve
     (fermion.Lattice.set_shift_freeze; 0, the default, is the reference's
@@ -188,12 +201,13 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
         seed = checkpoint.DEFAULT_SEED
     rng = RandomStream(seed)
     rng.rano(0)                      # y = rano(yran, idum) with idum = -1 (rhmc/cheisdwf.f:127-129)
+    outfile.header(os.path.join(workdir, "output"), seed, rng.seed(), L, lat.Lt, Ls, p, iter2, rhmc=True)
     sets = sets or {k: rz.load(k) for k in ("remez2", "remez4", "remez2g")}
     files = {k: open(os.path.join(workdir, k), "a") for k in
              ("fort.11", "fort.12", "fort.13", "fort.14", "control", "fort.5001", "fort.5011", "fort.5002",
               "fort.5012")}
     acc = dict(action=0.0, vel2=0.0, condmod=0.0, susc=0.0, cond=np.zeros(3), ancgm=0.0, yav=0.0, yyav=0.0,
-               naccp=0, ipbp=0, itot=0, it_pf=0, it_h=0)
+               naccp=0, ipbp=0, itot=0, it_pf=0, it_h=0, it_g=0, it_f=0)
     gaction = paction = action = 0.0
     pc, linfo = None, None
     if learned:
@@ -220,6 +234,8 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
             acc["itot"] += t["md_steps"]
             acc["it_pf"] += t["it_pf"]
             acc["it_h"] += t["it_h0"] + t["it_h1"]
+            acc["it_g"] += t["it_guidance"]
+            acc["it_f"] += t["it_force"]
             if isweep % iprint == 0:
                 cond, susc, condmod, susctot = aux_observables(chi)
                 acc["condmod"] += condmod
@@ -256,9 +272,16 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
                it_pseudofermion=acc["it_pf"] / n, it_acceptance=acc["it_h"] / (2 * n),
                it_measurement=acc["ancgm"] / ip, condmod=acc["condmod"] / ip, susc=acc["susc"] / ip,
                cond=(acc["cond"] / ip).tolist(), vel2=acc["vel2"] / n, action=acc["action"] / n)
-    with open(os.path.join(workdir, "output"), "a") as f:
-        for k, v in out.items():
-            f.write("%s %s\n" % (k, v))
+    # psibarpsi: the program never sets pbp (its estimator call is commented
+    # out, rhmc/cheisdwf.f:392); its build prints NaN for the average, and so
+    # does this file
+    outfile.averages(os.path.join(workdir, "output"),
+                     dict(naccp=acc["naccp"], itot=acc["itot"], yav=acc["yav"], yyav=acc["yyav"],
+                          guidance=acc["it_g"], force=acc["it_f"], acceptance=acc["it_h"],
+                          pseudofermion=acc["it_pf"], measurement=acc["ancgm"], ipbp=acc["ipbp"],
+                          condmod=acc["condmod"], susc=acc["susc"], cond=acc["cond"], pbp=float("nan"), vel2=acc["vel2"],
+                          action=acc["action"]),
+                     n, p["dt"], seed_end=rng.seed() if iwrite == 1 else None, rhmc=True)
     if linfo is not None:
         out["learned"] = linfo
     if iwrite == 1:

@@ -20,14 +20,21 @@ files, as the reference's own batch scripts lay it out (rhmc/scan_in_beta.sh);
 the fixture keeps what the program wrote: control (dH, dS and, for dwf3d,
 accepted), fort.11 (isweep, gauge and momentum action per site), fort.12/13/14
 (cond, susc, condmod and susctot), fort.5001/5011/5002/5012 (correlators),
-the totals of `output` (acceptances, mean trajectory length) and the final con
+the totals of `output` (acceptances, mean trajectory length) and its whole text
+(output_text: the run's header, residuals, seeds and averages in the
+program's own formats) and the final con
 (chi and the generator's seed), plus the inputs: the start con's chi and seed
 and the run parameters.  Nothing else is read from the programs: their
 trajectory lengths follow from the mean length, and the random stream's
 consumption from the final seed.
 
 Run in the build container:  python tests/golden/make_ref_traj.py [case ...]
-(OpenMP threads per program: CHS_REF_THREADS, default 8)
+(OpenMP threads per program: CHS_REF_THREADS, default 8).  The programs'
+OpenMP reductions combine their partial sums in an order that changes from
+run to run, so two runs of a case agree to the rounding level, not bit for
+bit (dwf3d, imass 3, two runs on 3 threads: dH within 2.2e-7, chi within
+6.8e-9; on 3 against 8 threads alike), with the same decisions, step counts
+and seed.
 """
 import os
 import resource
@@ -105,7 +112,9 @@ def run_case(name, threads=None):
         secs = time.time() - t0
         chi1, seed1, _ = checkpoint.read_con(os.path.join(d, "con"), kvol=12 ** 3, precision="fp64")
         naccp, atraj = _output_totals(os.path.join(d, "output"))
+        output_text = np.array(open(os.path.join(d, "output")).read())
         out = dict(chi0=chi0, seed0=seed0, con_precision=prec, chi1=chi1, seed1=seed1, naccp=naccp, atraj=atraj,
+                   output_text=output_text,
                    itot=int(round(atraj * p["iter2"] / p["dt"])), seconds=secs, threads=threads,
                    **{"par_" + k: v for k, v in p.items()})
         for f in ("control", "fort.11", "fort.12", "fort.13", "fort.14", "fort.5001", "fort.5011", "fort.5002",
