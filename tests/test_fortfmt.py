@@ -1,0 +1,65 @@
+"""The drivers' `output` writer (chiralheisenberg_amd/outfile.py over the
+edit descriptors of fortfmt.py) against the output files of the reference's
+own programs (tests/golden/ref_traj_rhmc_imass3.npz: dwf3d, imass 3;
+ref_traj_hmc.npz: the HMC sibling), CPU only: the header from the runs'
+parameters and seeds, and the averages block from the averages the programs
+printed, turned back into the sums the writer divides, must come out as the
+programs' text, character for character."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from chiralheisenberg_amd import fortfmt as F
+from chiralheisenberg_amd import outfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_NUM = re.compile(r"NaN|[-+]?(?:\d+\.\d*|\.\d+|\d+)(?:E[-+]\d+)?")
+
+
+def test_edit_descriptors():
+    assert F.e(1e-6, 9, 4) == ".1000E-05"            # the leading zero dropped to fit
+    assert F.e(1.061, 11, 4) == " 0.1061E+01"
+    assert F.e(-0.000591, 11, 3) == " -0.591E-03"
+    assert F.e(0.0, 10, 3) == " 0.000E+00"
+    assert F.e(float("nan"), 11, 3) == "        NaN"
+    assert F.f(0.06, 6, 4) == "0.0600" and F.f(247.61, 9, 3) == "  247.610"
+    assert F.f(0.5, 4, 3) == ".500" and F.f(12345.0, 6, 2) == "******"
+    assert F.i(7, 3) == "  7" and F.i(1234, 3) == "***"
+    assert F.list_real(3390816911360.0) == "3390816911360."
+
+
+@pytest.mark.parametrize("case,rhmc", [("rhmc_imass3", True), ("hmc", False)])
+def test_output_file_as_the_programs_write_it(case, rhmc, tmp_path):
+    g = np.load(os.path.join(HERE, "golden", "ref_traj_%s.npz" % case), allow_pickle=False)
+    ref = str(g["output_text"]).split("\n")
+    par = {k[4:]: g[k].item() for k in g.files if k.startswith("par_")}
+    par.update(rescgg=1e-6, rescga=1e-9)
+    seeds = [float(_NUM.findall(line)[0]) for line in ref if "seed:" in line]
+    n, itot, ip = int(par["iter2"]), int(g["itot"]), 1
+    path = str(tmp_path / "output")
+    outfile.header(path, seeds[0], seeds[1], 12, 12, 8, par, n, rhmc=rhmc)
+    block = "\n".join(ref[9:])
+    nums = [float(x) for x in _NUM.findall(block)]
+    # averages for last n / acceptances, length / exp-dH, error / guidance, acceptance [/ force, pseudofermion]
+    # / measurement / condmod, susc / cond (3) / psibarpsi / vel2, action / seed
+    k = 3
+    yav, err = nums[k], nums[k + 1]
+    k += 2
+    guid, acc_h = nums[k], nums[k + 1]
+    k += 2
+    force = pf = 0.0
+    if rhmc:
+        force, pf = nums[k], nums[k + 1]
+        k += 2
+    meas, condmod, susc = nums[k], nums[k + 1], nums[k + 2]
+    cond = nums[k + 3:k + 6]
+    pbp, vel2, action = nums[k + 6], nums[k + 7], nums[k + 8]
+    a = dict(naccp=int(nums[1]), itot=itot, yav=yav * n, yyav=n * (err * err * (n - 1) + yav * yav),
+             guidance=guid * itot, force=force * itot, acceptance=acc_h * 2 * n, pseudofermion=pf * n,
+             measurement=meas * ip, ipbp=ip, condmod=condmod, susc=susc, cond=cond, pbp=pbp, vel2=vel2 * n,
+             action=action * n)
+    outfile.averages(path, a, n, par["dt"], seed_end=seeds[2], rhmc=rhmc)
+    mine = open(path).read().split("\n")
+    assert mine == ref
