@@ -19,6 +19,7 @@ import numpy as np
 
 from . import fermion as fm
 from . import outfile
+from .fortfmt import list_directed as LD
 from .rhmc import RandomStream, aux_observables, heatbath_source
 
 DEFAULTS = dict(dt=0.06, beta=0.45, am=0.0, imass=1, iterl=16, itermax=1000, rescga=1e-9, rescgg=1e-6)
@@ -91,7 +92,8 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, ire
     """The sibling's main program: midout (unless par) and con in; control
     (dH, dS), fort.11 (isweep, gauge and momentum action per site), fort.12/13/14
     every iprint trajectories, con every icheck trajectories and at the end,
-    output (header, seeds and averages in the sibling's own formats).
+    output (header, seeds and averages in the sibling's own formats); the
+    data files hold list-directed records as the sibling writes them.
     force_precision: the force's CG, "fp64" (the
     reference CGNR) or the reliable-update CG in "fp32" or "fp16" form;
     hamilton stays FP64.  Returns the averages."""
@@ -127,11 +129,11 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, ire
             y = np.exp(t["dH"])
             acc["yav"] += y
             acc["yyav"] += y * y
-            files["control"].write("%.17g %.17g\n" % (t["dH"], t["dS"]))
+            files["control"].write(LD(t["dH"], t["dS"]) + "\n")
             if t["accepted"]:
                 acc["naccp"] += 1
                 action, gaction, paction = t["S1"] / kv, t["hg1"] / kv, t["hp1"] / kv
-            files["fort.11"].write("%d %.17g %.17g\n" % (isweep, gaction, paction))
+            files["fort.11"].write(LD(isweep, gaction, paction) + "\n")
             acc["action"] += action
             acc["vel2"] += t["vel2"]
             acc["itot"] += t["md_steps"]
@@ -143,9 +145,9 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, ire
                 acc["susc"] += susctot
                 acc["cond"] += cond
                 acc["ipbp"] += 1
-                files["fort.12"].write(" ".join("%.17g" % v for v in cond) + "\n")
-                files["fort.13"].write(" ".join("%.17g" % v for v in susc) + "\n")
-                files["fort.14"].write("%.17g %.17g\n" % (condmod, susctot))
+                files["fort.12"].write(LD(*cond) + "\n")
+                files["fort.13"].write(LD(*susc) + "\n")
+                files["fort.14"].write(LD(condmod, susctot) + "\n")
             if isweep % icheck == 0:
                 checkpoint.write_con(con, chi, rng.seed())
     finally:

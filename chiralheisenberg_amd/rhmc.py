@@ -21,6 +21,7 @@ import numpy as np
 
 from . import fermion as fm
 from . import outfile
+from .fortfmt import list_directed as LD
 from . import remez as rz
 from ._lib import Rng, lib
 
@@ -160,9 +161,9 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
     accepted), fort.12/13/14 (cond, susc, condmod and susctot every iprint
     trajectories), fort.5001/5011/5002/5012 (fermion correlators), con (every
     icheck trajectories and at the end) and output (header, seeds and
-    averages in the program's own formats 9001/9004/9022).  The other files'
-    numbers are written one record per line, space separated.  Returns the
-    averages.
+    averages in the program's own formats 9001/9004/9022).  The other files
+    hold one list-directed record per line, as the program's WRITE(unit,*)
+    prints it (fortfmt.list_directed).  Returns the averages.
 
     shift_freeze = f > 0 freezes converged shifts in every multi-shift solve
     (fermion.Lattice.set_shift_freeze; 0, the default, is the reference's
@@ -227,8 +228,8 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
             if t["accepted"]:
                 acc["naccp"] += 1
                 action, gaction, paction = t["S1"] / kv, t["hg1"] / kv, t["hp1"] / kv
-            files["control"].write("%.17g %.17g %d\n" % (t["dH"], t["dS"], t["accepted"]))
-            files["fort.11"].write("%d %.17g %.17g\n" % (isweep, gaction, paction))
+            files["control"].write(LD(t["dH"], t["dS"], int(t["accepted"])) + "\n")
+            files["fort.11"].write(LD(isweep, gaction, paction) + "\n")
             acc["action"] += action
             acc["vel2"] += t["vel2"]
             acc["itot"] += t["md_steps"]
@@ -241,9 +242,9 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
                 acc["condmod"] += condmod
                 acc["susc"] += susctot
                 acc["cond"] += cond
-                files["fort.12"].write(" ".join("%.17g" % v for v in cond) + "\n")
-                files["fort.13"].write(" ".join("%.17g" % v for v in susc) + "\n")
-                files["fort.14"].write("%.17g %.17g\n" % (condmod, susctot))
+                files["fort.12"].write(LD(*cond) + "\n")
+                files["fort.13"].write(LD(*susc) + "\n")
+                files["fort.14"].write(LD(condmod, susctot) + "\n")
                 ittt0 = int(lat.Lt * rng.rano(0)) + 1
                 form = "learned" if linfo and linfo["trained_after"] is not None else fermion_precision
                 c1, c2, its = fm.fermion(lat, chi, rescgm, ittt0, p["hpin"], p["imass"], form)
@@ -255,8 +256,8 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
                     pc.collect(0)
                 for it in range(lat.Lt):
                     for f, (u1, u2) in enumerate(((5001, 5011), (5002, 5012))):
-                        files["fort.%d" % u1].write("%d %.17g %.17g\n" % (it, c1[f, it].real, c1[f, it].imag))
-                        files["fort.%d" % u2].write("%d %.17g %.17g\n" % (it, c2[f, it].real, c2[f, it].imag))
+                        files["fort.%d" % u1].write(LD(it, c1[f, it].real, c1[f, it].imag) + "\n")
+                        files["fort.%d" % u2].write(LD(it, c2[f, it].real, c2[f, it].imag) + "\n")
                 acc["ancgm"] += its / 10.0
                 acc["ipbp"] += 1
             if isweep % icheck == 0:

@@ -22,7 +22,8 @@ accepted), fort.11 (isweep, gauge and momentum action per site), fort.12/13/14
 (cond, susc, condmod and susctot), fort.5001/5011/5002/5012 (correlators),
 the totals of `output` (acceptances, mean trajectory length) and its whole text
 (output_text: the run's header, residuals, seeds and averages in the
-program's own formats) and the final con
+program's own formats), the data files' text as written (text_control,
+text_fort_11, ...: the programs' list-directed records) and the final con
 (chi and the generator's seed), plus the inputs: the start con's chi and seed
 and the run parameters.  Nothing else is read from the programs: their
 trajectory lengths follow from the mean length, and the random stream's
@@ -120,6 +121,8 @@ def run_case(name, threads=None):
         for f in ("control", "fort.11", "fort.12", "fort.13", "fort.14", "fort.5001", "fort.5011", "fort.5002",
                   "fort.5012"):
             out[f.replace(".", "_")] = _numbers(os.path.join(d, f))
+            fp = os.path.join(d, f)
+            out["text_" + f.replace(".", "_")] = np.array(open(fp).read() if os.path.exists(fp) else "")
     dst = os.path.join(HERE, "ref_traj_%s.npz" % name)
     np.savez_compressed(dst, **out)
     print("%s: %d trajectories, %d accepted, mean length %.4f (%d MD steps), %.0f s; wrote %s" % (

@@ -28,6 +28,16 @@ def test_edit_descriptors():
     assert F.f(0.5, 4, 3) == ".500" and F.f(12345.0, 6, 2) == "******"
     assert F.i(7, 3) == "  7" and F.i(1234, 3) == "***"
     assert F.list_real(3390816911360.0) == "3390816911360."
+    # list-directed REAL*8 as the reference's flang build prints these values:
+    # the middle of the shortest candidates, not the correctly rounded digits
+    # (exact values ...8945236, ...4976542764, 2**54), and positional
+    # exactly for 0.095 <= |v| < 9.5e14
+    assert F.list_real(21.872546690894524) == "21.872546690894523"
+    assert F.list_real(-0.00020601407094976543) == "-2.0601407094976543E-04"
+    assert F.list_real(2.0 ** 54) == "1.8014398509481984E+16"
+    assert F.list_real(0.095) == ".095" and F.list_real(0.0949999999) == "9.49999999E-02"
+    assert F.list_real(949999990000000.0) == "949999990000000." and F.list_real(9.5e14) == "9.5E+14"
+    assert F.list_real(0.045) == "4.5E-02" and F.list_real(0.0) == "0." and F.list_real(1e-5) == "1.E-05"
 
 
 @pytest.mark.parametrize("case,rhmc", [("rhmc_imass3", True), ("hmc", False)])
@@ -63,3 +73,24 @@ def test_output_file_as_the_programs_write_it(case, rhmc, tmp_path):
     outfile.averages(path, a, n, par["dt"], seed_end=seeds[2], rhmc=rhmc)
     mine = open(path).read().split("\n")
     assert mine == ref
+
+
+_INT = re.compile(r"^-?\d+$")
+
+
+@pytest.mark.parametrize("case", ["rhmc_imass1", "rhmc_imass3", "hmc"])
+def test_data_files_as_the_programs_write_them(case):
+    """Every record of the programs' data files (control, fort.11-14,
+    fort.50xx: WRITE(unit,*) lists of integers and REAL*8), read back to its
+    numbers and written again by list_directed, is the program's line
+    character for character."""
+    g = np.load(os.path.join(HERE, "golden", "ref_traj_%s.npz" % case), allow_pickle=False)
+    keys = [k for k in g.files if k.startswith("text_")]
+    assert keys
+    n = 0
+    for k in keys:
+        for line in str(g[k]).splitlines():
+            vals = [int(t) if _INT.match(t) else float(t) for t in line.split()]
+            assert F.list_directed(*vals) == line, (k, line)
+            n += 1
+    assert n >= 5

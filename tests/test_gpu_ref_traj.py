@@ -52,6 +52,7 @@ CASES = [c for c in ("rhmc_imass1", "rhmc_imass3", "hmc")
 
 
 _NUM = re.compile(r"NaN|[-+]?(?:\d+\.\d*|\.\d+|\d+)(?:E[-+]\d+)?")
+_INT = re.compile(r"^-?\d+$")
 
 
 def _compare_output(mine, ref):
@@ -67,6 +68,22 @@ def _compare_output(mine, ref):
                 continue
             fx, fy = float(x), float(y)
             assert (math.isnan(fx) and math.isnan(fy)) or abs(fx - fy) <= rtol * max(abs(fy), 1e-3), (a, b)
+
+
+def _same_records(path, ref_text):
+    """The data file's records are laid out as the program's: the same
+    lines, one blank before every item, the same number of items, the
+    integer items (isweep, accepted, the correlator's t) the same text."""
+    mine = open(path).read().splitlines() if os.path.exists(path) else []
+    ref = ref_text.splitlines()
+    assert len(mine) == len(ref)
+    for a, b in zip(mine, ref):
+        ta, tb = a.split(), b.split()
+        assert len(ta) == len(tb) and a.startswith(" ") and "  " not in a, (a, b)
+        for x, y in zip(ta, tb):
+            assert (_INT.match(x) is not None) == (_INT.match(y) is not None), (a, b)
+            if _INT.match(y):
+                assert x == y, (a, b)
 
 
 def _rows(path):
@@ -116,6 +133,9 @@ def test_driver_matches_reference_program(case, tmp_path):
             assert np.array_equal(mine[:, 0], ref[:, 0])
             # measured 2.2e-6 of the largest entry (imass = 3)
             assert np.abs(mine[:, 1:] - ref[:, 1:]).max() <= 1e-5 * np.abs(ref[:, 1:]).max()
+    for k in g.files:
+        if k.startswith("text_"):
+            _same_records(os.path.join(d, k[5:].replace("_", ".")), str(g[k]))
     if "output_text" in g.files:
         _compare_output(open(os.path.join(d, "output")).read(), str(g["output_text"]))
     chi1, seed1, _ = checkpoint.read_con(os.path.join(d, "con"), kvol=12 ** 3, precision="fp64")
