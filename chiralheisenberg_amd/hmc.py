@@ -93,7 +93,8 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, ire
     (dH, dS), fort.11 (isweep, gauge and momentum action per site), fort.12/13/14
     every iprint trajectories, con every icheck trajectories and at the end,
     output (header, seeds and averages in the sibling's own formats); the
-    data files hold list-directed records as the sibling writes them.
+    data files hold list-directed records as the sibling writes them and,
+    like output, start afresh on every run (dwf3d's rhmc.run appends).
     force_precision: the force's CG, "fp64" (the
     reference CGNR) or the reliable-update CG in "fp32" or "fp16" form;
     hamilton stays FP64.  Returns the averages."""
@@ -117,7 +118,10 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, ire
     rng = RandomStream(seed)
     rng.rano(0)                      # y = rano(yran, idum) with idum = -1
     outfile.header(os.path.join(workdir, "output"), seed, rng.seed(), L, lat.Lt, Ls, p, iter2, rhmc=False)
-    files = {k: open(os.path.join(workdir, k), "a") for k in ("fort.11", "fort.12", "fort.13", "fort.14", "control")}
+    # the sibling opens control with status='unknown' and units 11-14
+    # implicitly, without position='append' (mh/cheisdwf_hmc.f:74): its flang
+    # build starts these files afresh on every run (dwf3d appends to its own)
+    files = {k: open(os.path.join(workdir, k), "w") for k in ("fort.11", "fort.12", "fort.13", "fort.14", "control")}
     acc = dict(action=0.0, vel2=0.0, condmod=0.0, susc=0.0, cond=np.zeros(3), yav=0.0, yyav=0.0, naccp=0, ipbp=0,
                itot=0, it_f=0, it_h=0)
     gaction = paction = action = 0.0

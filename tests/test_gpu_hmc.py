@@ -139,6 +139,11 @@ def test_hmc_run_files(tmp_path):
     assert len(open(tmp_path / "fort.14").readlines()) == 3
     chi, seed, _ = checkpoint.read_con(tmp_path / "con", kvol=64)
     assert np.all(np.isfinite(chi)) and seed != 693764523745.0
+    # a second run from the con the first wrote: the sibling's build starts
+    # control and fort.11-14 afresh (no position='append', mh/cheisdwf_hmc.f:74)
+    hmc.run(str(tmp_path), 4, 4, iprint=1, iter2=2)
+    assert len(open(tmp_path / "control").readlines()) == 2
+    assert len(open(tmp_path / "fort.11").readlines()) == 2
 
 
 def _hmc_md_start(seed):

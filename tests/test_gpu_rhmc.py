@@ -118,6 +118,15 @@ def test_driver_run_matches_oracle(tmp_path):
     assert seed_end == orng.get()
     assert out["acceptances"] == sum(a for _, a in control)
     assert len(open(tmp_path / "fort.14").readlines()) == iter2 // iprint
+    # a second run from the con the first wrote: dwf3d appends to control,
+    # fort.11-14 and the correlators (position='append', rhmc/cheisdwf.f:100-113)
+    # and starts output afresh (two seed lines in its header, one at the end)
+    n5001 = len(open(tmp_path / "fort.5001").readlines())
+    rhmc.run(str(tmp_path), L, Ls, iter2=iprint, iprint=iprint, rescgm=1e-10)
+    assert len(open(tmp_path / "control").readlines()) == iter2 + iprint
+    assert len(open(tmp_path / "fort.11").readlines()) == iter2 + iprint
+    assert len(open(tmp_path / "fort.5001").readlines()) == n5001 + L
+    assert open(tmp_path / "output").read().count("seed:") == 3
 
 
 def _md_start(L, Ls, seed):
