@@ -139,6 +139,21 @@ struct GraphGuard {
     }
 };
 
+// device allocations of one call (the timing helpers' buffers), freed on every
+// return, error paths included (hipFree waits for kernels still using them)
+struct DevAllocs {
+    std::vector<void *> p;
+    hipError_t alloc(void **out, size_t bytes) {
+        *out = nullptr;
+        const hipError_t e = hipMalloc(out, bytes);
+        if (e == hipSuccess) p.push_back(*out);
+        return e;
+    }
+    ~DevAllocs() {
+        for (void *q : p) (void)hipFree(q);
+    }
+};
+
 int ensure(chs_ctx *c, chs_ctx::Buf &b, size_t bytes) {
     if (b.n >= bytes) return CHS_OK;
     if (b.p) HIPCHK(c, hipFree(b.p));
@@ -1437,12 +1452,13 @@ int chs_time_dslash(chs_ctx *c, int prec, int dagger, int nbuf, int iters, doubl
     if (!prec_ok(prec) || nbuf < 1 || iters < 1) return fail(c, CHS_EINVAL, "bad arguments");
     const size_t bytes = (size_t)c->kferm * prec_bytes(prec);
     std::vector<void *> ins(nbuf), outs(nbuf);
+    DevAllocs mem;
     double *chi = nullptr;
-    HIPCHK(c, hipMalloc(&chi, 24 * c->kvol));
+    HIPCHK(c, mem.alloc((void **)&chi, 24 * c->kvol));
     fill_random_real(c, chi, 3L * c->kvol, 1);
     for (int i = 0; i < nbuf; i++) {
-        HIPCHK(c, hipMalloc(&ins[i], bytes));
-        HIPCHK(c, hipMalloc(&outs[i], bytes));
+        HIPCHK(c, mem.alloc(&ins[i], bytes));
+        HIPCHK(c, mem.alloc(&outs[i], bytes));
         fill_random(c, ins[i], prec, c->kferm, 2 + i);   // random bits: zeros run ~15% faster
     }
     HIPCHK(c, hipGetLastError());
@@ -1454,11 +1470,6 @@ int chs_time_dslash(chs_ctx *c, int prec, int dagger, int nbuf, int iters, doubl
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *avg_ms = ms / iters;
-    for (int i = 0; i < nbuf; i++) {
-        hipFree(ins[i]);
-        hipFree(outs[i]);
-    }
-    hipFree(chi);
     return CHS_OK;
 }
 
@@ -1470,9 +1481,10 @@ static int time_copy_impl(chs_ctx *c, int prec, int nbuf, int iters, long pgrid,
     if (c->kferm >= (1L << 32)) return fail(c, CHS_EINVAL, "spinor too large for the copy kernel");
     const size_t bytes = (size_t)c->kferm * prec_bytes(prec);
     std::vector<void *> ins(nbuf), outs(nbuf);
+    DevAllocs mem;
     for (int i = 0; i < nbuf; i++) {
-        HIPCHK(c, hipMalloc(&ins[i], bytes));
-        HIPCHK(c, hipMalloc(&outs[i], bytes));
+        HIPCHK(c, mem.alloc(&ins[i], bytes));
+        HIPCHK(c, mem.alloc(&outs[i], bytes));
         fill_random(c, ins[i], prec, c->kferm, 2 + i);
     }
     const long n = c->kferm;
@@ -1498,10 +1510,6 @@ static int time_copy_impl(chs_ctx *c, int prec, int nbuf, int iters, long pgrid,
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *avg_ms = ms / iters;
-    for (int i = 0; i < nbuf; i++) {
-        hipFree(ins[i]);
-        hipFree(outs[i]);
-    }
     return CHS_OK;
 }
 
@@ -1523,14 +1531,15 @@ int chs_time_cg_stencil(chs_ctx *c, int prec, int nbuf, int iters, double hpin, 
     // (FP64 for FP64, FP32 otherwise), as in chs_solve_mdagm_dev
     const int pacc = prec == CHS_FP64 ? CHS_FP64 : CHS_FP32;
     std::vector<void *> a(4 * nbuf, nullptr);
+    DevAllocs mem;
     double *chi = nullptr;
-    HIPCHK(c, hipMalloc(&chi, 24 * c->kvol));
+    HIPCHK(c, mem.alloc((void **)&chi, 24 * c->kvol));
     fill_random_real(c, chi, 3L * c->kvol, 1);
     HIPCHK(c, hipMemsetAsync(c->dscal, 0, sizeof(double), c->stream));   // alpha
     unsigned long long seed = 2;
     for (size_t k = 0; k < a.size(); k++) {
         const int pk = (k % 4) < 2 ? prec : pacc;
-        HIPCHK(c, hipMalloc(&a[k], (size_t)c->kferm * prec_bytes(pk)));
+        HIPCHK(c, mem.alloc(&a[k], (size_t)c->kferm * prec_bytes(pk)));
         fill_random(c, a[k], pk, c->kferm, seed++);   // random bits: zeros run ~15% faster
     }
     HIPCHK(c, hipGetLastError());
@@ -1550,8 +1559,6 @@ int chs_time_cg_stencil(chs_ctx *c, int prec, int nbuf, int iters, double hpin, 
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *avg_ms = ms / iters;
-    for (void *p : a) hipFree(p);
-    hipFree(chi);
     return CHS_OK;
 }
 
