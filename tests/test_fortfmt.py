@@ -94,3 +94,31 @@ def test_data_files_as_the_programs_write_them(case):
             assert F.list_directed(*vals) == line, (k, line)
             n += 1
     assert n >= 5
+
+
+FLANG = "/opt/rocm/llvm/bin/flang"
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not installed")
+def test_list_real_against_flang(tmp_path):
+    """list_real against the flang runtime itself (the compiler the
+    reference is built with): a five-line program reads each value and
+    writes it list-directed; 20,000 values over 60 decades, the powers of
+    two, integers, and the edges of the positional range."""
+    import subprocess
+    rs = np.random.RandomState(5)
+    vals = list(rs.randn(12000) * 10.0 ** rs.randint(-30, 30, 12000)) + list(rs.rand(4000)) + \
+        list(rs.rand(4000) * 100) + [2.0 ** k for k in range(-70, 70)] + \
+        list(rs.randint(-10 ** 6, 10 ** 6, 200).astype(float)) + \
+        [0.1, 0.095, 0.0949999999, 9.5e14, 949999990000000.0, 1e15, 1e16, 2.0 ** 54, 5e-324, 1.7976931348623157e308,
+         np.nextafter(0.095, 0.0), np.nextafter(9.5e14, 0.0), -0.0, 0.0]
+    src = tmp_path / "ld.f"
+    src.write_text("      program ld\n      real*8 v\n      integer k, n\n      read(5,*) n\n"
+                   "      do k = 1, n\n        read(5,*) v\n        write(6,*) v\n      enddo\n      end\n")
+    exe = tmp_path / "ld"
+    subprocess.run([FLANG, "-fdefault-real-8", str(src), "-o", str(exe)], check=True, capture_output=True)
+    inp = "%d\n" % len(vals) + "".join("%r\n" % float(v) for v in vals)
+    out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, check=True).stdout.splitlines()
+    assert len(out) == len(vals)
+    bad = [(v, o, F.list_real(v)) for v, o in zip(vals, out) if o != " " + F.list_real(v)]
+    assert not bad, bad[:5]
