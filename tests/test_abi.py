@@ -46,6 +46,18 @@ def test_no_gpu_fails_loudly():
         fm.Lattice(4, 4)
 
 
+@pytest.mark.parametrize("dims,msg", [((1, 4, None), "extents"), ((4, 1, None), "extents"), ((4, 4, 1), "extents"),
+                                      ((1024, 256, None), "too large")])
+def test_context_rejects_bad_extents(dims, msg):
+    """chs_ctx_create checks the extents before it touches a device: an
+    extent below 2, and a lattice whose 8 Ls kvol components overflow the
+    kernels' 32-bit indexing."""
+    from chiralheisenberg_amd import fermion as fm
+    L, Ls, Lt = dims
+    with pytest.raises(_lib.ChsError, match=msg):
+        fm.Lattice(L, Ls, Lt=Lt)
+
+
 def test_rng_matches_oracle_on_host():
     """The product's host generator (libchs) and the oracle draw the same
     stream (both pinned to the reference by test_oracle)."""
