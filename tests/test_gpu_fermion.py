@@ -44,7 +44,7 @@ def test_fermion_vs_reference_golden():
         lat.close()
 
 
-@pytest.mark.parametrize("L,Ls,Lt,imass,ittt0", [(6, 4, 10, 1, 3), (8, 6, 8, 3, 8)])
+@pytest.mark.parametrize("L,Ls,Lt,imass,ittt0", [(6, 4, 10, 1, 3), (8, 6, 8, 3, 8), (5, 3, 6, 1, 6)])
 def test_fermion_vs_oracle(L, Ls, Lt, imass, ittt0):
     lat = fm.Lattice(L, Ls, Lt)
     olat = orc.Lattice(L, Ls, Lt)

@@ -80,6 +80,44 @@ def test_force_parity(imass):
     assert err < 1e-10, (err, itg, itc)
 
 
+@pytest.mark.parametrize("L,Ls,Lt", [(5, 6, 6), (6, 3, 4)])
+@pytest.mark.parametrize("imass", [1, 3])
+def test_force_hamilton_other_shapes(L, Ls, Lt, imass):
+    """An odd L, Lt != L and an odd Ls: force and Hamiltonian (all three
+    parts) against the oracle to 1e-10 relative at solver tolerance 1e-12,
+    with the pinning field on."""
+    lat, olat = lattices(L, Ls, Lt)
+    rs = np.random.RandomState(100 + L + Ls + imass)
+    phi, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    Fg, _, itg = fm.force(lat, phi, 1e-12, chi, 0.475, R2G, 0.0, 0.2, imass)
+    Fc, _, itc = orc.force(olat, phi, 1e-12, chi, 0.475, R2G, 0.2, imass)
+    err = np.abs(Fg - Fc).max() / np.abs(Fc).max()
+    assert err < 1e-10, (err, itg, itc)
+    pp = rs.randn(3, lat.kvol)
+    hg, _, _ = fm.hamilton(lat, phi, 1e-12, chi, pp, 0.475, R2, hpin=0.2, imass=imass)
+    hc, _, _ = orc.hamilton(olat, phi, 1e-12, chi, pp, 0.475, R2, 0.2, imass)
+    for a, b in zip(hg, hc):
+        assert abs(a - b) <= 1e-10 * abs(b), (hg, hc)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("L,Ls,Lt", [(5, 6, 6), (6, 3, 4)])
+def test_solve_mdagm_other_shapes(prec, L, Ls, Lt):
+    """Every form of the M^dagger M solve on an odd L (the one-site FP32
+    stencil instead of the paired one), Lt != L and an odd Ls: the FP64 true
+    residual, recomputed with the oracle's operator, meets 1e-10."""
+    lat, olat = lattices(L, Ls, Lt)
+    rs = np.random.RandomState(110 + L + Ls)
+    b, chi = spinor(rs, lat.shape), phi_field(rs, lat.kvol)
+    bd = fm.spinor_dev(lat, b)
+    xd = fm.spinor_dev(lat, np.zeros(lat.shape, np.complex128))
+    st = fm.solve_mdagm(lat, bd, xd, fm.DeviceArray.from_numpy(lat, chi), 1e-10, prec)
+    x = fm.to_host(lat, xd)
+    r = b - orc.dslash(olat, orc.dslash(olat, x, chi, 0.0, 1, False), chi, 0.0, 1, True)
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    assert rel <= 1.05e-10, (prec, rel, st)
+
+
 @pytest.mark.parametrize("imass", [1, 3])
 def test_force_fp32_solves(imass):
     """Force with FP32-storage multi-shift solves (FP64 Lanczos scalars,
