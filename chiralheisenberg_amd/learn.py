@@ -6,7 +6,7 @@ P(r) = c u + C1^T C2^T C2 C1 u (u = r / rms(r); C1 1x1x1, 16 Ls -> 16
 channels; C2 3x3x3 over x, y, t, 16 -> 16), BF16 MFMA kernels for inference
 and training.  Training data are (source, solution) pairs of M^dagger M x = b
 produced by the plain solver, the loss the A-norm of P's error; the trained
-network is the preconditioner of flexible CG, whose answer is judged by the
+network is the preconditioner of flexible CG, whose answer is accepted on the
 FP64 true residual like every other solve (plain CG remains the fallback).
 
 Pairs come either from pairs_dev (Gaussian sources at one chi) or from the

@@ -353,7 +353,7 @@ int chs_nn_pair_dev(chs_ctx *ctx, int i, const float **src, const float **sol, c
  * same pairs when they share one chi */
 int chs_nn_train_collected(chs_ctx *ctx, int epochs, float lr, double hpin, int imass, double *epoch_loss);
 /* Flexible CG (FP32 iteration, FP64 reliable updates every delta of
- * residual reduction, answer judged by the FP64 true relative residual) with
+ * residual reduction, answer accepted on the FP64 true relative residual) with
  * z = P(r) (use_nn = 1) or z = r (use_nn = 0, plain CG).  Plain-CG fallback:
  * when with the network |r|^2 has not fallen by a factor 8 within 64
  * iterations the solve continues from its current x as plain CG
