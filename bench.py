@@ -427,6 +427,43 @@ def force_bench(fm, lat, chi, beta, res=1e-6, seed=5, forms=("fp64", "fp32"), fr
     return out
 
 
+TRAJ_FORMS = {"reference": dict(force_precision="fp64", shift_freeze=0.0),
+              "fast": dict(force_precision="fp32", shift_freeze=0.01)}
+
+
+def trajectory_bench(L, Ls, beta, forms=("reference", "fast"), seed=693764523745.0):
+    """One RHMC trajectory (rhmc.trajectory: heatbath, two Hamiltonians, a
+    force per MD step; dt 0.05, iterl 20, the reference's tolerances rescga
+    1e-9 / rescgg 1e-6, imass 1) at L, Ls from the same unthermalised field
+    (chi ~ N(0, 1/beta), seeded) and generator seed, in the reference's solves
+    (FP64 everywhere) and in the opt-in fast forms (FP32 force storage, shift
+    freezing f = 0.01): seconds, MD steps, H0, dH, decision of each."""
+    from chiralheisenberg_amd import fermion as fm, remez, rhmc
+    par = dict(dt=0.05, beta=beta, am=0.0, hpin=0.0, imass=1, iterl=20, itermax=1000, iprint=5, rescga=1e-9,
+               rescgg=1e-6)
+    sets = {k: remez.load(k) for k in ("remez2", "remez4", "remez2g")}
+    out = {"L": L, "Ls": Ls, "dt": par["dt"], "iterl": par["iterl"]}
+    for name in forms:
+        kw = TRAJ_FORMS[name]
+        lat = fm.Lattice(L, Ls)
+        fm.set_force_precision(lat, kw["force_precision"])
+        lat.set_shift_freeze(kw["shift_freeze"])
+        chi = np.random.RandomState(8).randn(3, lat.kvol) / np.sqrt(beta)
+        print("bench: trajectory L=%d Ls=%d, %s forms %s" % (L, Ls, name, kw), file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        t = rhmc.trajectory(lat, chi, rhmc.RandomStream(seed), 1, par, sets)
+        lat.synchronize()
+        secs = time.perf_counter() - t0
+        lat.close()
+        out[name] = dict(kw, seconds=secs, md_steps=int(t["md_steps"]), H0=float(t["H0"]), dH=float(t["dH"]),
+                         accepted=int(t["accepted"]))
+        print("bench: %s %.1f s, %d MD steps, dH %.6f" % (name, secs, t["md_steps"], t["dH"]), file=sys.stderr,
+              flush=True)
+    if "reference" in out and "fast" in out:
+        out["speedup"] = out["reference"]["seconds"] / out["fast"]["seconds"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -451,6 +488,9 @@ def main():
                     help="only the pseudofermion force figures (FP64 / FP32, with and without shift freezing)")
     ap.add_argument("--force-forms", default="fp64,fp32", help="with --force-only: storage forms, comma-separated")
     ap.add_argument("--force-freeze", default="0,0.01", help="with --force-only: shift-freezing factors")
+    ap.add_argument("--trajectory", action="store_true",
+                    help="only one RHMC trajectory at --L/--Ls, reference and fast forms (one JSON line)")
+    ap.add_argument("--traj-forms", default="reference,fast", help="with --trajectory: reference, fast")
     ap.add_argument("--learned-pairs", type=int, default=200,
                     help="training pairs of the learned-preconditioner figure")
     ap.add_argument("--learned-epochs", type=int, default=2)
@@ -462,6 +502,10 @@ def main():
 
     if args.sweep:
         print(json.dumps(sweep_bench(args.tol, args.Ls)), flush=True)
+        return
+    if args.trajectory:
+        forms = tuple(args.traj_forms.split(","))
+        print(json.dumps({"trajectory": trajectory_bench(args.L, args.Ls, args.beta, forms)}), flush=True)
         return
     if args.batch:
         out = {"%s_%s" % (m, p): batch_bench(m, precision=p) for m in BATCH_MIXES for p in ("fp64", "fp32")}
