@@ -85,3 +85,15 @@ def test_driver_options_checked_before_any_gpu_work(tmp_path):
     from chiralheisenberg_amd import rhmc
     with pytest.raises(ValueError):
         rhmc.run(str(tmp_path), 4, 4, learned=dict(pairs=0))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says so,
+    before it touches a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr and p.stdout == ""
