@@ -464,6 +464,47 @@ def trajectory_bench(L, Ls, beta, forms=("reference", "fast"), seed=693764523745
     return out
 
 
+def shard_bench(L, Lt_full, Ls, shards):
+    """The hot kernels at the shape of one t-slab shard of an L x L x
+    Lt_full lattice split over `shards` GPUs (L x L x Lt_full/shards, the
+    same Ls), against the whole lattice on one GPU, in the same process: the
+    FP64 dslash, the binary16 dslash, the binary16 fused CG stencil and a
+    plain FP64 copy, each over enough buffer sets to run from HBM where the
+    sets can exceed the Infinity Cache.  The halo a shard would exchange
+    per stencil and face (Ls L^2 8 complex FP64) and its time on one xGMI
+    link are stated beside it; no second GPU is used."""
+    from chiralheisenberg_amd import fermion as fm
+    if Lt_full % shards:
+        raise ValueError("Lt %d does not split into %d slabs" % (Lt_full, shards))
+    out = {"L": L, "Lt": Lt_full, "Ls": Ls, "shards": shards}
+    for name, Lt in (("whole", Lt_full), ("shard", Lt_full // shards)):
+        lat = fm.Lattice(L, Ls, Lt=Lt)
+        kvol = L * L * Lt
+        V = kvol * Ls * 8 * 16
+        V32, V16, C = V // 2, V // 4, 3 * kvol * 8
+        r = {"Lt": Lt, "spinor_MB": V / 1e6}
+        for key, prec, nbytes, inb in (("dslash_fp64", "fp64", 2 * V + C, V), ("dslash_fp16", "fp16", 2 * V16 + C, V16)):
+            ms = fm.time_dslash(lat, prec, False, nbuf=cold_nbuf(inb), iters=200)
+            r[key] = {"us": ms * 1e3, "GB_per_s": nbytes / (ms * 1e-3) / 1e9, "frac": nbytes / (ms * 1e-3) / HBM_PEAK}
+        ms = fm.time_cg_stencil(lat, "fp16", nbuf=cold_nbuf(V16 + V32), iters=200)
+        nb = V16 + 2 * V32 + C
+        r["cg_stencil_fp16"] = {"us": ms * 1e3, "GB_per_s": nb / (ms * 1e-3) / 1e9, "frac": nb / (ms * 1e-3) / HBM_PEAK}
+        ms = fm.time_copy(lat, "fp64", nbuf=cold_nbuf(V), iters=200)
+        r["copy_fp64"] = {"us": ms * 1e3, "GB_per_s": 2 * V / (ms * 1e-3) / 1e9, "frac": 2 * V / (ms * 1e-3) / HBM_PEAK}
+        lat.close()
+        out[name] = r
+        print("bench: %s (L=%d Lt=%d Ls=%d): dslash fp64 %.2f us, fp16 %.2f us, cg stencil fp16 %.2f us, copy %.2f us"
+              % (name, L, Lt, Ls, r["dslash_fp64"]["us"], r["dslash_fp16"]["us"], r["cg_stencil_fp16"]["us"],
+                 r["copy_fp64"]["us"]), file=sys.stderr, flush=True)
+    for k in ("dslash_fp64", "dslash_fp16", "cg_stencil_fp16", "copy_fp64"):
+        # the shard's share of the whole lattice's time over the shard's time
+        out[k + "_efficiency"] = out["whole"][k]["us"] / shards / out["shard"][k]["us"]
+    halo = Ls * L * L * 8 * 16
+    out["halo_per_face"] = {"bytes": halo, "us_one_xgmi_link": halo / 153e9 * 1e6,
+                            "what": "Ls L^2 8 complex FP64 per face and stencil, at ~153 GB/s per link"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -488,6 +529,8 @@ def main():
                     help="only the pseudofermion force figures (FP64 / FP32, with and without shift freezing)")
     ap.add_argument("--force-forms", default="fp64,fp32", help="with --force-only: storage forms, comma-separated")
     ap.add_argument("--force-freeze", default="0,0.01", help="with --force-only: shift-freezing factors")
+    ap.add_argument("--shard", type=int, default=0,
+                    help="only the hot kernels at the shape of one of N t-slab shards of the --L lattice (one JSON line)")
     ap.add_argument("--trajectory", action="store_true",
                     help="only one RHMC trajectory at --L/--Ls, reference and fast forms (one JSON line)")
     ap.add_argument("--traj-forms", default="reference,fast", help="with --trajectory: reference, fast")
@@ -502,6 +545,9 @@ def main():
 
     if args.sweep:
         print(json.dumps(sweep_bench(args.tol, args.Ls)), flush=True)
+        return
+    if args.shard:
+        print(json.dumps({"shard": shard_bench(args.L, args.L, args.Ls, args.shard)}), flush=True)
         return
     if args.trajectory:
         forms = tuple(args.traj_forms.split(","))
