@@ -26,6 +26,18 @@ def load(name_or_path):
     return (np.array([a0] + a), np.array(ad), np.array([b0] + b), np.array(bd))
 
 
+def load_sets(workdir=None):
+    """The three sets of a run: the caller's own files remez2, remez4 and
+    remez2g in workdir when they are there -- dwf3d reads them from its
+    working directory (rhmc/cheisdwf.f:114-116, :140-159) -- else the copies
+    bundled here."""
+    out = {}
+    for k in ("remez2", "remez4", "remez2g"):
+        p = os.path.join(workdir, k) if workdir else None
+        out[k] = load(p if p and os.path.exists(p) else k)
+    return out
+
+
 def evaluate(coef, x):
     """Value of the partial fraction a0 + sum a_k / (x + s_k) at x."""
     num, den = coef

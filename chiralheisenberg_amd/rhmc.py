@@ -156,7 +156,10 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
         rescgm=1e-9, force_precision="fp64", fermion_precision="fp64", sets=None, learned=None, shift_freeze=0.0):
     """The main program dwf3d (rhmc/cheisdwf.f:85-460) with every fermion
     routine on the GPU.  workdir holds the inputs (midout unless par gives
-    the parameters, con when iread = 1) and receives the reference's outputs:
+    the parameters, con when iread = 1, and the rational approximations
+    remez2 / remez4 / remez2g as the program reads them there, the bundled
+    copies where they are absent, unless sets gives them) and receives the
+    reference's outputs:
     fort.11 (isweep, gauge and momentum action per site), control (dH, dS,
     accepted), fort.12/13/14 (cond, susc, condmod and susctot every iprint
     trajectories), fort.5001/5011/5002/5012 (fermion correlators), con (every
@@ -203,7 +206,7 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
     rng = RandomStream(seed)
     rng.rano(0)                      # y = rano(yran, idum) with idum = -1 (rhmc/cheisdwf.f:127-129)
     outfile.header(os.path.join(workdir, "output"), seed, rng.seed(), L, lat.Lt, Ls, p, iter2, rhmc=True)
-    sets = sets or {k: rz.load(k) for k in ("remez2", "remez4", "remez2g")}
+    sets = sets or rz.load_sets(workdir)
     files = {k: open(os.path.join(workdir, k), "a") for k in
              ("fort.11", "fort.12", "fort.13", "fort.14", "control", "fort.5001", "fort.5011", "fort.5002",
               "fort.5012")}

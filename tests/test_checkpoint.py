@@ -50,3 +50,20 @@ def test_sread_default_and_errors(tmp_path):
         ck.read_con(os.path.join(G, "con_l12_fp64"), kvol=8 ** 3)
     with pytest.raises(ValueError):   # 20,736 bytes: FP64 chi(864,3) or FP32 chi(1728,3)
         ck.read_con(os.path.join(G, "con_l12_fp32"))
+
+
+def test_run_reads_the_callers_remez_files(tmp_path):
+    """rhmc.run takes remez2 / remez4 / remez2g from its working directory
+    when they are there, as dwf3d does (rhmc/cheisdwf.f:114-116), and the
+    bundled copies otherwise."""
+    from chiralheisenberg_amd import remez
+    own = remez.load("remez2g")
+    text = "%.17e\n" % (2 * own[0][0]) + "".join("%.17e, %.17e\n" % (a, d) for a, d in zip(own[0][1:], own[1]))
+    text += "%.17e\n" % own[2][0] + "".join("%.17e, %.17e\n" % (b, d) for b, d in zip(own[2][1:], own[3]))
+    (tmp_path / "remez2g").write_text(text)
+    sets = remez.load_sets(str(tmp_path))
+    assert sets["remez2g"][0][0] == 2 * own[0][0]
+    assert np.array_equal(sets["remez2g"][1], own[1]) and np.array_equal(sets["remez2g"][3], own[3])
+    for k in ("remez2", "remez4"):
+        assert all(np.array_equal(x, y) for x, y in zip(sets[k], remez.load(k)))
+    assert all(np.array_equal(x, y) for x, y in zip(remez.load_sets(None)["remez2g"], own))
