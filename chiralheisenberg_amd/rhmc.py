@@ -20,6 +20,7 @@ import os
 import numpy as np
 
 from . import fermion as fm
+from ._lib import NoConvergence
 from . import outfile
 from .fortfmt import list_directed as LD
 from . import remez as rz
@@ -65,6 +66,19 @@ def heatbath_source(lat, rng):
     return R
 
 
+def carry_on(call):
+    """The reference's multi-shift solve ends at its iteration cap with
+    the last iterate and no message (rhmc/cheisdwf.f:603,639): run call()
+    and, where the library reports the cap (NoConvergence with the results
+    it kept), go on with those results; any other error stands."""
+    try:
+        return call()
+    except NoConvergence as e:
+        if getattr(e, "result", None) is None:
+            raise
+        return e.result
+
+
 def leapfrog(lat, Phi, chit, pp, par, remez2g, stop, its=None):
     """The molecular-dynamics evolution of rhmc/cheisdwf.f:261-315, in place
     on chit and pp: a half momentum step, then steps of chit += dt pp and the
@@ -78,7 +92,7 @@ def leapfrog(lat, Phi, chit, pp, par, remez2g, stop, its=None):
     args = (par["rescgg"], chit, par["beta"], remez2g, par["am"], par["hpin"], par["imass"])
 
     def force():
-        F, _, (i1, i2) = fm.force(lat, Phi, *args)
+        F, _, (i1, i2) = carry_on(lambda: fm.force(lat, Phi, *args))
         if its is not None:
             its[0] += i1
             its[1] += i2
@@ -109,19 +123,19 @@ def trajectory(lat, chi, rng, isweep, par=None, sets=None, measure_draw=True):
     kv = lat.kvol
     chit = chi.copy()
     R = heatbath_source(lat, rng)
-    Phi, it_pf = fm.qmrherm(lat, R, p["rescga"], a4, ad4, chit, p["am"], p["hpin"], p["imass"], 0)
+    Phi, it_pf = carry_on(lambda: fm.qmrherm(lat, R, p["rescga"], a4, ad4, chit, p["am"], p["hpin"], p["imass"], 0))
     pp = np.empty((3, kv))
     ps = rng.gaussp(kv)
     pp[0], pp[1] = ps[0], ps[1]
     ps = rng.gaussp(kv)
     pp[2] = ps[0]
-    (H0, hg0, hp0, S0), _, it_h0 = fm.hamilton(lat, Phi, p["rescga"], chit, pp, p["beta"], sets["remez2"],
-                                               p["am"], p["hpin"], p["imass"], isweep)
+    (H0, hg0, hp0, S0), _, it_h0 = carry_on(lambda: fm.hamilton(lat, Phi, p["rescga"], chit, pp, p["beta"],
+                                                                sets["remez2"], p["am"], p["hpin"], p["imass"], isweep))
     proby = 1.0 / p["iterl"]
     its = [0, 0]
     steps = leapfrog(lat, Phi, chit, pp, p, sets["remez2g"], lambda it: rng.rano(1) < proby, its)
-    (H1, hg1, hp1, S1), _, it_h1 = fm.hamilton(lat, Phi, p["rescga"], chit, pp, p["beta"], sets["remez2"],
-                                               p["am"], p["hpin"], p["imass"], isweep)
+    (H1, hg1, hp1, S1), _, it_h1 = carry_on(lambda: fm.hamilton(lat, Phi, p["rescga"], chit, pp, p["beta"],
+                                                                sets["remez2"], p["am"], p["hpin"], p["imass"], isweep))
     dH = H0 - H1
     accepted = rng.rano(1) < np.exp(dH)
     if accepted:

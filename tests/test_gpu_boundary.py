@@ -153,3 +153,33 @@ def test_mirror_dslash_moves_only_the_result(lats):
     for out, dag in ((out1, False), (out2, True)):
         ref = orc.dslash(olat, R, chi, 0.2, 1, dag)
         assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_trajectory_carries_on_at_the_lanczos_cap():
+    """With the multi-shift cap lowered below what the solves need, the
+    trajectory runs to its end on the last iterates, as the reference's does
+    (rhmc/cheisdwf.f:639: no message, no stop), and matches the oracle's
+    trajectory under the same cap: the same steps and decision, dH within
+    1e-6."""
+    from chiralheisenberg_amd import rhmc
+    L, Ls, cap, seed = 4, 4, 40, 693764523745.0
+    lat, olat = fm.Lattice(L, Ls), orc.Lattice(L, Ls)
+    par = dict(dt=0.1, beta=0.475, am=0.0, hpin=0.0, imass=1, iterl=3, itermax=1000, iprint=5, rescga=1e-9,
+               rescgg=1e-6)
+    sets = {"remez2": R2, "remez4": R4, "remez2g": R2G}
+    chi0 = np.random.RandomState(9).randn(3, L ** 3) / np.sqrt(par["beta"])
+    lat.set_lanczos_cap(cap)
+    orc.set_lanczos_cap(cap)
+    try:
+        chi_g, chi_c = chi0.copy(), chi0.copy()
+        g = rhmc.trajectory(lat, chi_g, rhmc.RandomStream(seed), 1, par, sets)
+        c = orc.rhmc(olat, par, orc.Rng(seed), chi_c, 1, 1, sets)[0]
+    finally:
+        lat.set_lanczos_cap(0)
+        orc.set_lanczos_cap(0)
+        lat.close()
+    print("capped at %d: heatbath %d iterations, steps %d / %d, dH %.8f / %.8f" % (
+        cap, g["it_pf"], g["md_steps"], c["md_steps"], g["dH"], c["dH"]))
+    assert g["it_pf"] == cap                      # the heatbath solve did stop at the cap
+    assert g["md_steps"] == c["md_steps"] and g["accepted"] == c["accepted"]
+    assert abs(g["dH"] - c["dH"]) <= 1e-6 * max(1.0, abs(c["dH"]))
