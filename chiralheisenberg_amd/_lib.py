@@ -21,6 +21,7 @@ class ChsError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("libchs error %d: %s" % (code, msg))
         self.code = code
+        self.msg = msg
 
 
 class NoConvergence(ChsError):

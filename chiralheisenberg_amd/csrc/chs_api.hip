@@ -899,7 +899,7 @@ int chs_fermion(chs_ctx *c, const double *chi, double res, double hpin, int imas
     HIPCHK(c, hipMemcpyAsync(dchi, chi, 24 * c->kvol, hipMemcpyHostToDevice, c->stream));
     std::vector<double> c1(4 * Lt, 0.0), c2(4 * Lt, 0.0);   // [f][t] (re, im)
     std::vector<double2> S[2];
-    int iter = 0, status = CHS_OK;
+    int iter = 0, ncap = 0;
     for (int ks = 1; ks <= nsource; ks++) {
         const int ittt = (ittt0 + ks * (Lt / nsource)) % Lt + 1;
         for (int ids = 0; ids < 2; ids++) {
@@ -927,7 +927,7 @@ int chs_fermion(chs_ctx *c, const double *chi, double res, double hpin, int imas
                                              0, &st);
                 it = st.iterations;
             }
-            if (rc == CHS_ENOCONV) status = rc;   // the reference carries on with the last iterate
+            if (rc == CHS_ENOCONV) ncap++;   // the reference carries on with the last iterate
             else if (rc) return rc;
             if (prec != CHS_LEARNED && (rc = nn_store_pair(c, phi, x, dchi))) return rc;
             iter += it;
@@ -965,7 +965,10 @@ int chs_fermion(chs_ctx *c, const double *chi, double res, double hpin, int imas
         cferm2[k] = c2[k] / nsource;
     }
     if (iterations) *iterations = iter;
-    return status;
+    // the message begins with the number of solves at the cap: the reference
+    // writes its congrad message once for each (rhmc/cheisdwf.f:1151-1153)
+    if (ncap) return fail(c, CHS_ENOCONV, "%d of the %d propagator solves exceeded the iteration cap", ncap, 2 * nsource);
+    return CHS_OK;
 }
 
 int chs_congrad(chs_ctx *c, const double *phi, double res, int *itercg, double am, double hpin, int imass,
@@ -1086,6 +1089,7 @@ int chs_hmc_force(chs_ctx *c, const double *phi, double *x0, double res, double 
         it = st.iterations;
     }
     if (rc && rc != CHS_ENOCONV) return rc;   // the sibling, too, carries on at its iteration cap
+    const bool capped = rc == CHS_ENOCONV;
     // x0 = 2 X1 - x0 (mh/cheisdwf_hmc.f:434-438)
     hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(kBlock), 0, c->stream, n, 2.0, (const double2 *)dx1, -1.0,
                        (double2 *)dx0);
@@ -1099,6 +1103,9 @@ int chs_hmc_force(chs_ctx *c, const double *phi, double *x0, double res, double 
     if ((rc = d2h_spinor(c, x0, dx0))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (itercg) *itercg = it;
+    // the force and x0 are written; the sibling writes its congrad message
+    // and carries on (mh/cheisdwf_hmc.f:705-707)
+    if (capped) return fail(c, CHS_ENOCONV, "1 of the 1 force solves exceeded the iteration cap");
     return CHS_OK;
 }
 
@@ -1124,6 +1131,7 @@ int chs_hmc_hamilton(chs_ctx *c, const double *phi, double *x0, double res, doub
     int it = 0;
     rc = chs_congrad_dev(c, dphi, res, &it, 0.0, imass, dchi, dx);
     if (rc && rc != CHS_ENOCONV) return rc;
+    const bool capped = rc == CHS_ENOCONV;
     reduce_launch(c, c->kferm, dphi, dx, 1, c->dscal + 10);
     double sums[3];
     HIPCHK(c, hipMemcpyAsync(sums, c->dscal + 8, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1135,6 +1143,7 @@ int chs_hmc_hamilton(chs_ctx *c, const double *phi, double *x0, double res, doub
     h4[2] = hp;
     h4[3] = hg + hf;
     if (itercg) *itercg = it;
+    if (capped) return fail(c, CHS_ENOCONV, "1 of the 1 Hamiltonian solves exceeded the iteration cap");
     return CHS_OK;
 }
 

@@ -207,8 +207,9 @@ def fermion(lat, chi, res, ittt0, hpin=0.0, imass=1, precision="fp64"):
     c2 = np.zeros_like(c1)
     it = ctypes.c_int(0)
     form = _lib.CHS_LEARNED if precision == "learned" else PRECISIONS[precision]
-    check(lat.h, lib().chs_fermion(lat.h, _p(lat._field(chi)), res, hpin, imass, int(ittt0), form,
-                                   _p(c1), _p(c2), ctypes.byref(it)))
+    rc = lib().chs_fermion(lat.h, _p(lat._field(chi)), res, hpin, imass, int(ittt0), form, _p(c1), _p(c2),
+                           ctypes.byref(it))
+    _check_keep(lat, rc, (c1, c2, it.value))
     return c1, c2, it.value
 
 
@@ -220,8 +221,9 @@ def hmc_force(lat, Phi, x0, res, chi, beta, imass=1, am=0.0):
     assert x0.dtype == np.complex128 and x0.flags.c_contiguous and x0.shape == lat.shape
     F = np.empty((3, lat.kvol))
     it = ctypes.c_int(0)
-    check(lat.h, lib().chs_hmc_force(lat.h, _p(Phi), _p(x0), res, am, imass, _p(lat._field(chi)), beta, _p(F),
-                                     ctypes.byref(it)))
+    rc = lib().chs_hmc_force(lat.h, _p(Phi), _p(x0), res, am, imass, _p(lat._field(chi)), beta, _p(F),
+                             ctypes.byref(it))
+    _check_keep(lat, rc, (F, it.value))
     return F, it.value
 
 
@@ -232,8 +234,9 @@ def hmc_hamilton(lat, Phi, x0, res, chi, pp, beta, imass=1, am=0.0):
     assert x0.dtype == np.complex128 and x0.flags.c_contiguous and x0.shape == lat.shape
     h4 = np.empty(4)
     it = ctypes.c_int(0)
-    check(lat.h, lib().chs_hmc_hamilton(lat.h, _p(Phi), _p(x0), res, am, imass, _p(lat._field(chi)),
-                                        _p(lat._field(pp)), beta, _p(h4), ctypes.byref(it)))
+    rc = lib().chs_hmc_hamilton(lat.h, _p(Phi), _p(x0), res, am, imass, _p(lat._field(chi)), _p(lat._field(pp)),
+                                beta, _p(h4), ctypes.byref(it))
+    _check_keep(lat, rc, (tuple(h4), it.value))
     return tuple(h4), it.value
 
 

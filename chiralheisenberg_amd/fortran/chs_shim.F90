@@ -237,9 +237,12 @@ subroutine congrad(Phi, res, itercg, am, hpin, imass)
   integer(c_int) :: it, rc
   call chs_ensure(real(am3, c_double), ibound)
   rc = c_congrad(ctx, Phi, real(res, c_double), it, real(am, c_double), real(hpin, c_double), int(imass, c_int), chi, x)
-  ! like the reference, running into the iteration cap only prints a message
+  ! like the reference, running into the iteration cap only writes its
+  ! message, to the terminal and to the program's output file (unit 7,
+  ! rhmc/cheisdwf.f:1151-1153)
   if (rc == 3) then
     write(6, 1000)
+    write(7, 1000)
 1000 format(' # iterations of congrad exceeds niterc')
   else
     call chs_check(rc, 'congrad')

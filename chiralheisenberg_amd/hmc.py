@@ -19,13 +19,29 @@ import numpy as np
 
 from . import fermion as fm
 from . import outfile
+from ._lib import NoConvergence
 from .fortfmt import list_directed as LD
 from .rhmc import RandomStream, aux_observables, heatbath_source
 
 DEFAULTS = dict(dt=0.06, beta=0.45, am=0.0, imass=1, iterl=16, itermax=1000, rescga=1e-9, rescgg=1e-6)
 
 
-def leapfrog(lat, Phi, x0, chit, pp, par, stop):
+def _congrad(call, output):
+    """The sibling's congrad keeps its last iterate at the iteration cap
+    and writes its message to output (mh/cheisdwf_hmc.f:705-707): run
+    call() and, where the library reports the cap, go on with the results
+    it kept, the message appended to output when given."""
+    try:
+        return call()
+    except NoConvergence as e:
+        if getattr(e, "result", None) is None:
+            raise
+        if output:
+            outfile.congrad_capped(output, e)
+        return e.result
+
+
+def leapfrog(lat, Phi, x0, chit, pp, par, stop, output=None):
     """The sibling's molecular-dynamics evolution (mh/cheisdwf_hmc.f:140-330),
     in place on chit, pp and the chronological guess x0: a half momentum
     step, then steps of chit += dt pp and the force; after each step's force
@@ -33,11 +49,13 @@ def leapfrog(lat, Phi, x0, chit, pp, par, stop):
     full one when itermax ends the loop).  The sibling's stop is one
     rano(yrun) < 1/iterl draw per step.  Returns (steps, force iterations)."""
     dt = par["dt"]
-    F, its = fm.hmc_force(lat, Phi, x0, par["rescgg"], chit, par["beta"], par["imass"])
+    def force():
+        return _congrad(lambda: fm.hmc_force(lat, Phi, x0, par["rescgg"], chit, par["beta"], par["imass"]), output)
+    F, its = force()
     pp -= 0.5 * dt * F
     for step in range(1, par["itermax"] + 1):
         chit += dt * pp
-        F, it = fm.hmc_force(lat, Phi, x0, par["rescgg"], chit, par["beta"], par["imass"])
+        F, it = force()
         its += it
         if stop(step):
             pp -= 0.5 * dt * F
@@ -46,10 +64,12 @@ def leapfrog(lat, Phi, x0, chit, pp, par, stop):
     return par["itermax"], its
 
 
-def trajectory(lat, chi, rng, isweep, par=None):
+def trajectory(lat, chi, rng, isweep, par=None, output=None):
     """One HMC trajectory; chi (float64 (3, kvol)) is replaced in place when
     the proposal is accepted.  Returns a dict of H0, H1, dH, accepted, the
-    Hamiltonian parts, the mean square velocity and iteration counts."""
+    Hamiltonian parts, the mean square velocity and iteration counts.
+    output: the file that receives the congrad message of a solve at its
+    iteration cap (run() passes its output file)."""
     p = dict(DEFAULTS)
     if par:
         p.update(par)
@@ -63,10 +83,12 @@ def trajectory(lat, chi, rng, isweep, par=None):
     pp[0], pp[1] = ps[0], ps[1]
     ps = rng.gaussp(kv)
     pp[2] = ps[0]
-    (H0, hg0, hp0, S0), it_h0 = fm.hmc_hamilton(lat, Phi, x0, p["rescga"], chit, pp, p["beta"], p["imass"])
+    (H0, hg0, hp0, S0), it_h0 = _congrad(lambda: fm.hmc_hamilton(lat, Phi, x0, p["rescga"], chit, pp, p["beta"],
+                                                                  p["imass"]), output)
     proby = 1.0 / p["iterl"]
-    steps, its = leapfrog(lat, Phi, x0, chit, pp, p, lambda step: rng.rano(1) < proby)
-    (H1, hg1, hp1, S1), it_h1 = fm.hmc_hamilton(lat, Phi, x0, p["rescga"], chit, pp, p["beta"], p["imass"])
+    steps, its = leapfrog(lat, Phi, x0, chit, pp, p, lambda step: rng.rano(1) < proby, output)
+    (H1, hg1, hp1, S1), it_h1 = _congrad(lambda: fm.hmc_hamilton(lat, Phi, x0, p["rescga"], chit, pp, p["beta"],
+                                                                  p["imass"]), output)
     dH = H0 - H1
     accepted = True
     if dH < 0.0:
@@ -127,7 +149,7 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=100, ire
     gaction = paction = action = 0.0
     try:
         for isweep in range(1, iter2 + 1):
-            t = trajectory(lat, chi, rng, isweep, p)
+            t = trajectory(lat, chi, rng, isweep, p, os.path.join(workdir, "output"))
             if isweep == 1:
                 action, gaction, paction = t["S0"] / kv, t["hg0"] / kv, t["hp0"] / kv
             y = np.exp(t["dH"])

@@ -31,6 +31,21 @@ def header(path, seed_in, seed_now, L, Lt, Ls, p, iter2, rhmc=True):
         fh.write("\n".join(lines) + "\n")
 
 
+CONGRAD_CAP = " # iterations of congrad exceeds niterc"   # format 1000 of both programs' congrad
+
+
+def congrad_capped(path, err):
+    """Append the programs' congrad message (rhmc/cheisdwf.f:1151-1153,
+    mh/cheisdwf_hmc.f:705-707) once for each solve that a NoConvergence
+    from the library reports at its iteration cap (the message's leading
+    count)."""
+    msg = getattr(err, "msg", "") or ""
+    head = msg.split()[0] if msg.split() else ""
+    n = int(head) if head.isdigit() else 1
+    with open(path, "a") as fh:
+        fh.write((CONGRAD_CAP + "\n") * n)
+
+
 def _div(a, b):
     with np.errstate(divide="ignore", invalid="ignore"):
         return float(np.float64(a) / np.float64(b))

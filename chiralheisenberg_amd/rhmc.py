@@ -264,7 +264,13 @@ def run(workdir, L, Ls, Lt=None, par=None, iter2=None, iprint=5, icheck=None, ir
                 files["fort.14"].write(LD(condmod, susctot) + "\n")
                 ittt0 = int(lat.Lt * rng.rano(0)) + 1
                 form = "learned" if linfo and linfo["trained_after"] is not None else fermion_precision
-                c1, c2, its = fm.fermion(lat, chi, rescgm, ittt0, p["hpin"], p["imass"], form)
+                try:
+                    c1, c2, its = fm.fermion(lat, chi, rescgm, ittt0, p["hpin"], p["imass"], form)
+                except NoConvergence as e:   # congrad at its cap: the message, then the last iterates
+                    if getattr(e, "result", None) is None:
+                        raise
+                    outfile.congrad_capped(os.path.join(workdir, "output"), e)
+                    c1, c2, its = e.result
                 if linfo and linfo["trained_after"] is None and pc.num_collected() >= int(learned["pairs"]):
                     linfo["pairs"] = pc.num_collected()
                     linfo["losses"] = pc.train_collected(int(learned.get("epochs", 1)), float(learned.get("lr", 1e-3)),

@@ -206,11 +206,17 @@ int chs_hamilton(chs_ctx *ctx, const double *phi, double *h, double *hg,
  * CHS_FP16 call appends its (source, solution) pair to it.
  * cferm1 = fort.5001/5002, cferm2 = fort.5011/5012: complex (re, im)
  * [flavour][t], t fastest, 2*Lt complex each.  iterations = total solver
- * iterations (the reference's aviter times 10). */
+ * iterations (the reference's aviter times 10).  A solve at its iteration
+ * cap keeps its last iterate, as the reference's congrad does; the results
+ * are written and CHS_ENOCONV returned, chs_last_error() beginning with the
+ * number of such solves (the reference writes its message once for each). */
 int chs_fermion(chs_ctx *ctx, const double *chi, double res, double hpin, int imass, int ittt0, int prec,
                 double *cferm1, double *cferm2, int *iterations);
 
 /* ---- HMC sibling mh/cheisdwf_hmc.f (its operator is M with hpin = 0) ---- */
+/* Both calls, like the sibling's congrad, keep the last iterate at the
+ * iteration cap: the results are written and CHS_ENOCONV is returned,
+ * chs_last_error() beginning with the number of capped solves (1). */
 /* Replaces force(Phi,X0,dSdchi,iflag,res1,am,imass), mh/cheisdwf_hmc.f:393:
  * X1 = (M^dagger M)^{-1} Phi by congrad from the guess x0 (/vector/ in the
  * reference), dSdchi = 4 beta chi - 2 (wall bilinears of X1 and M X1), and

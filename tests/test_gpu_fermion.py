@@ -72,3 +72,47 @@ def test_fermion_rejects_bad_source_slice():
     with pytest.raises(ChsError):
         fm.fermion(lat, chi, 1e-8, 5)
     lat.close()
+
+
+def test_fermion_congrad_cap_carries_on():
+    """L=Lt=2, Ls=2: congrad's cap Ls*kvol = 16 lies below what res 1e-11
+    needs, so solves stop there.  The library keeps their last iterates, as
+    the reference's congrad does (rhmc/cheisdwf.f:1081,1151), writes the
+    correlators and reports how many solves stopped (NoConvergence, its
+    message's leading count, the results attached); they equal the oracle's,
+    which carries on the same way."""
+    from chiralheisenberg_amd._lib import NoConvergence
+    lat, olat = fm.Lattice(2, 2), orc.Lattice(2, 2)
+    chi = phi_field(np.random.RandomState(3), lat.kvol)
+    with pytest.raises(NoConvergence) as ei:
+        fm.fermion(lat, chi, 1e-11, 1, 0.3, 1)
+    c1, c2, it = ei.value.result
+    ncap = int(ei.value.msg.split()[0])
+    o1, o2, oit = orc.fermion(olat, chi, 1e-11, 1, 0.3, 1)
+    print("%d of 10 solves at the cap; iterations %d (oracle %d)" % (ncap, it, oit))
+    assert 1 <= ncap <= 10 and it <= 10 * 16
+    assert np.abs(c1 - o1).max() <= 1e-10 * np.abs(o1).max()
+    assert np.abs(c2 - o2).max() <= 1e-10 * np.abs(o2).max()
+    lat.close()
+
+
+def test_drivers_write_the_congrad_message(tmp_path):
+    """rhmc.run and hmc.run at L=Lt=2, Ls=2, where congrad stops at its cap:
+    both runs end normally, and their output holds the programs' message
+    ' # iterations of congrad exceeds niterc' between the header and the
+    averages, once per capped solve."""
+    from chiralheisenberg_amd import checkpoint, hmc, outfile, rhmc
+    rs = np.random.RandomState(4)
+    chi0 = rs.randn(3, 8) / np.sqrt(0.475)
+    for name, run, mid in (("rhmc", rhmc.run, "0.1 0.475 1.0 0.00 0.0 1 3 1\n"),
+                           ("hmc", hmc.run, "0.1 0.475 1.0 0.00 1 3 1\n")):
+        d = tmp_path / name
+        d.mkdir()
+        (d / "midout").write_text(mid)
+        checkpoint.write_con(d / "con", chi0, 122110675517440.0)
+        run(str(d), 2, 2, iprint=1)
+        lines = open(d / "output").read().split("\n")
+        k = [i for i, l in enumerate(lines) if l == outfile.CONGRAD_CAP]
+        first_avg = next(i for i, l in enumerate(lines) if "averages for last" in l)
+        print("%s: %d congrad messages" % (name, len(k)))
+        assert k and 9 <= min(k) and max(k) < first_avg
